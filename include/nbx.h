@@ -1,0 +1,173 @@
+/*
+ * nbx.h -- C ABI of the MI355X (gfx950) Barnes-Hut disk step.
+ *
+ * Drop-in for the reference's step-loop call surface (ualiberik/nbody-sim):
+ * every entry point below names the reference function it replaces.  The
+ * ABI uses plain pointers and sizes only; all particle pointers are DEVICE
+ * pointers (HBM) unless a name says otherwise.
+ *
+ * Conventions
+ *  - Return value: 0 on success, a negative NBX_E_* code on failure.  The
+ *    message of the last failure on the calling thread is nbx_last_error()
+ *    (the reference throws std::runtime_error from nbody_check_cuda,
+ *    ParticleData.h:9-12; the C++ mirror in nbx.hpp rethrows it).
+ *  - Every entry point returns after its work has completed on the device
+ *    (the reference synchronises after each launch, BHTree.cuh:42), unless
+ *    nbx_tree_set_async() switched the tree's stream to asynchronous mode.
+ *  - Units: G = 1, M_sun = 1, AU, T0 = 1/(2 pi) yr (Perturber.h:13-18).
+ *  - Not re-entrant on one nbx_tree: concurrent calls on the same tree race.
+ */
+#ifndef NBX_H
+#define NBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NBX_OK 0
+#define NBX_E_ARG (-1)      /* invalid argument (null pointer, n mismatch, n <= 0)    */
+#define NBX_E_HIP (-2)      /* HIP runtime error (launch, copy, allocation)           */
+#define NBX_E_OVERFLOW (-3) /* a walk stack or node pool overflowed (no silent loss)  */
+#define NBX_E_STATE (-4)    /* call order violated (e.g. force before build)          */
+
+/* ParticleData (include/ParticleData.h:19-32): SoA, 10 float arrays + id.
+ * Positions AU, velocities AU/T0, accelerations AU/T0^2, mass M_sun. */
+typedef struct nbx_particles {
+    float *x, *y, *z;
+    float *vx, *vy, *vz;
+    float *ax, *ay, *az;
+    float *mass;
+    int32_t *id;
+    int32_t n;
+} nbx_particles;
+
+/* OctreeData (include/BHTree.cuh:13-36): opaque.  Owns the tree scratch
+ * (keys, order, nodes, collision deltas), a HIP stream and, for the fused
+ * step, a captured HIP graph. */
+typedef struct nbx_tree nbx_tree;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nbx_last_error(void);
+
+/* ---- device memory for ParticleData (ParticleData.h:35-123) -------------- */
+/* allocateParticlesGPU (ParticleData.h:61-75): all 11 arrays, zeroed. */
+int nbx_particles_alloc(nbx_particles *p, int32_t n);
+/* freeParticlesGPU (ParticleData.h:77-87). */
+int nbx_particles_free(nbx_particles *p);
+/* copyToGPU (ParticleData.h:90-105): host -> device, all 11 arrays. */
+int nbx_copy_to_device(const nbx_particles *host, nbx_particles *dev);
+/* copyToCPU (ParticleData.h:108-123): device -> host, all 11 arrays. */
+int nbx_copy_to_host(const nbx_particles *dev, nbx_particles *host);
+
+/* ---- tree lifetime (BHTree.cu:358-420) ----------------------------------- */
+/* allocateOctree (BHTree.cu:358-381). */
+int nbx_tree_create(nbx_tree **t, int32_t n_bodies);
+/* Batched variant (no reference counterpart; the perturber-sweep batching):
+ * n_seg independent disks stored back to back in one nbx_particles, disk s
+ * holding seg_counts[s] bodies.  Every tree entry point then processes all
+ * disks in one launch per phase; disk s never interacts with disk s'. */
+int nbx_tree_create_batch(nbx_tree **t, int32_t n_seg, const int32_t *seg_counts);
+/* freeOctree (BHTree.cu:383-395). */
+int nbx_tree_destroy(nbx_tree *t);
+/* resetOctree (BHTree.cu:397-420): clears the collision deltas; the Morton
+ * build needs no pool reset. */
+int nbx_tree_reset(nbx_tree *t);
+/* Stream control: `stream` is a hipStream_t (NULL = the tree's own stream).
+ * async=1 makes entry points return without waiting for the device. */
+int nbx_tree_set_stream(nbx_tree *t, void *stream);
+int nbx_tree_set_async(nbx_tree *t, int async);
+int nbx_tree_sync(nbx_tree *t);
+
+/* ---- tree build (BHTree.cuh:43-46) --------------------------------------- */
+/* launchBBoxKernel (BHTree.cu:422-452): AABB, 1.001 padding and the root
+ * cube of BHTree.cu:187-191, computed on the device. */
+int nbx_bbox(nbx_tree *t, const nbx_particles *p);
+/* launchBuildKernel (BHTree.cu:454-499): 63-bit keys from the root-cube
+ * octant descent, stable radix sort, hierarchy from the sorted keys. */
+int nbx_build(nbx_tree *t, const nbx_particles *p);
+/* launchSummarizeKernel (BHTree.cu:501-516): node mass / COM, full
+ * bottom-up reduction (no pass cap). */
+int nbx_summarize(nbx_tree *t);
+/* launchSortKernel (BHTree.cu:518-525): the leaf order.  The Morton sort of
+ * nbx_build already is the DFS order; this call only validates the tree. */
+int nbx_sort(nbx_tree *t);
+/* OctreeData::sort_idx (BHTree.cuh:20): copy the body order to dev_out[n]. */
+int nbx_sort_idx(nbx_tree *t, int32_t *dev_out);
+/* Morton keys of the bodies in sorted order (dev_out[n], uint64). */
+int nbx_sorted_keys(nbx_tree *t, uint64_t *dev_out);
+
+/* ---- forces and collisions (BHTree.cuh:47-50, Perturber.h:73-75) --------- */
+/* launchForceKernel (BHTree.cu:600-609): a = star term + tree self-gravity,
+ * written with '='.  theta is the half-width / COM-distance ratio of
+ * BHTree.cu:577.  self_a (optional, device [3][n] or NULL) receives the
+ * tree part alone. */
+int nbx_force(nbx_tree *t, nbx_particles *p, float theta, float softening, float star_mass);
+int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, float star_mass,
+                 float *self_ax, float *self_ay, float *self_az);
+/* No reference counterpart (diagnostic): per-body counts of accepted records
+ * (self excluded) and opened internal records of the theta walk on the last
+ * summarized tree, written to device arrays in original body order. */
+int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n_open);
+/* launchCollisionKernel (BHTree.cu:688-708): exact pair set
+ * dist^2 < r_coll^2, impulse J = -(1+e) dvn / 2 for approaching pairs,
+ * deltas accumulated per body in a fixed order, then v += dv. */
+int nbx_collide(nbx_tree *t, nbx_particles *p, float r_coll, float restitution);
+/* launchPerturberForceKernel (Perturber.cu:148-160): a += softened pull. */
+int nbx_perturber_force(nbx_particles *p, float px, float py, float pz, float pmass,
+                        float softening);
+
+/* ---- integrator (Integrator.cuh:9-22) ------------------------------------ */
+int nbx_half_kick(nbx_particles *p, float dt_half);          /* Integrator.cu:80-85  */
+int nbx_drift(nbx_particles *p, float dt);                   /* Integrator.cu:87-92  */
+int nbx_reset_acceleration(nbx_particles *p);                /* Integrator.cu:94-98  */
+int nbx_gas_drag(nbx_particles *p, float drag_rate, float dt,
+                 float star_mass);                           /* Integrator.cu:100-109 */
+
+/* ---- Friends-of-Friends (Aggregate.h:17-18) ------------------------------ */
+/* Partition of detectAggregates (Aggregate.cpp:26-76) for link length ll on
+ * the device: dev_labels[i] = smallest particle index in i's group (1-body
+ * groups label themselves).  Builds its own tree from p. */
+int nbx_fof(nbx_tree *t, const nbx_particles *p, float link_length, int32_t *dev_labels);
+
+/* ---- fused step (main.cu:101-140) ---------------------------------------- */
+typedef struct nbx_step_params {
+    float dt, theta, softening, star_mass;
+    float restitution, gas_drag_rate;
+    int32_t collisions;      /* 1 = two-pass collision impulse each step          */
+    int32_t use_graph;       /* 1 = replay one captured hipGraph per step         */
+} nbx_step_params;
+
+/* Per-disk values (one per disk of a batched tree, one for a plain tree). */
+typedef struct nbx_disk_params {
+    float r_coll;            /* collision_radius_factor * particlePhysicalRadius */
+    int32_t perturber_enabled;
+    float pert_mass_msun, pert_radius_au, pert_phase0, pert_omega;
+} nbx_disk_params;
+
+/* Initial forces of main.cu:85-96 (tree, star, perturber at t=0). */
+int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                    const nbx_disk_params *dp);
+/* n_steps KDK steps of main.cu:101-140 starting at step index step0
+ * (sim_time after step k is float(k+1)*dt).  Perturber positions follow
+ * updatePerturber (Perturber.h:60-66) evaluated on the host. */
+int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+             const nbx_disk_params *dp, int32_t step0, int32_t n_steps);
+
+/* ---- diagnostics --------------------------------------------------------- */
+/* Tree statistics of the last build: number of octree nodes and maximum
+ * depth per disk (host arrays of length n_seg, may be NULL). */
+int nbx_tree_stats(nbx_tree *t, int32_t *n_nodes, int32_t *max_level);
+/* Average duration (ms) of the named phase's kernels over the last
+ * nbx_step call when timing was enabled with nbx_tree_set_timing(t, 1).
+ * Phases: "kdk1", "build", "walk", "collide", "kdk2".  Returns 0 and
+ * writes *ms, or NBX_E_ARG for an unknown phase. */
+int nbx_tree_set_timing(nbx_tree *t, int on);
+int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBX_H */

@@ -1,0 +1,632 @@
+// nbx_abi.hip -- the extern "C" entry points of include/nbx.h.
+//
+// Host-side sequencing only: argument checks, device allocation, launch
+// order of the gfx950 kernels, error propagation (a failed HIP call or a
+// kernel error flag becomes a negative return code and a message), and the
+// fused step loop of main.cu:101-140.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/nbx.h"
+#include "nbx_launch.hpp"
+
+using namespace nbx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(NBX_E_HIP, "HIP error in %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+enum Stage { kNone = 0, kBBox = 1, kBuilt = 2, kSummarized = 3 };
+
+}  // namespace
+
+struct nbx_tree {
+    SegTable T{};
+    Bufs B{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool async = false;
+    int stage = kNone;
+    int *h_err = nullptr;  // pinned copy of B.err
+    bool timing = false;
+    std::vector<hipEvent_t> ev;  // timing events: pairs per phase
+    float phase_ms[5] = {0, 0, 0, 0, 0};
+    int phase_n = 0;
+};
+
+namespace {
+
+const char *kPhases[5] = {"kdk1", "build", "walk", "collide", "kdk2"};
+
+int finish(nbx_tree *t)
+{
+    HIPCHK(hipGetLastError());
+    if (t->async) return NBX_OK;
+    HIPCHK(hipMemcpyAsync(t->h_err, t->B.err, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                          t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    if (t->h_err[0]) {  // report once, then clear for the next call
+        launch_fill_int(t->B.err, 0, 1, t->stream);
+        HIPCHK(hipStreamSynchronize(t->stream));
+    }
+    if (t->h_err[0] & kErrStack)
+        return fail(NBX_E_OVERFLOW, "tree walk stack overflow (more than %d pending blocks)",
+                    kStackCap);
+    if (t->h_err[0] & kErrTree) return fail(NBX_E_STATE, "inconsistent octree (internal error)");
+    return NBX_OK;
+}
+
+int finish_plain()
+{
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return NBX_OK;
+}
+
+bool particles_ok(const nbx_particles *p)
+{
+    return p && p->n > 0 && p->x && p->y && p->z && p->vx && p->vy && p->vz && p->ax && p->ay &&
+           p->az && p->mass;
+}
+
+template <class T>
+int dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void **)p, count * sizeof(T)));
+    return NBX_OK;
+}
+
+int alloc_bufs(nbx_tree *t)
+{
+    Bufs &B = t->B;
+    const int n = t->T.n_total;
+    B.n = n;
+    B.n_tiles = (n + kSortTile - 1) / kSortTile;
+    int rc = NBX_OK;
+#define ALLOC(ptr, cnt) \
+    if ((rc = dmalloc(&(ptr), (size_t)(cnt))) != NBX_OK) return rc
+    ALLOC(B.keys[0], n);
+    ALLOC(B.keys[1], n);
+    ALLOC(B.vals[0], n);
+    ALLOC(B.vals[1], n);
+    ALLOC(B.hist, 256 * B.n_tiles);
+    ALLOC(B.bodies, n);
+    ALLOC(B.left, n);
+    ALLOC(B.right, n);
+    ALLOC(B.par_int, n);
+    ALLOC(B.par_leaf, n);
+    ALLOC(B.plen, n);
+    ALLOC(B.flags, n);
+    ALLOC(B.cnt, n);
+    ALLOC(B.first, n);
+    ALLOC(B.alloc, kMaxSeg);
+    ALLOC(B.bin_com, n);
+    B.n_rec = t->T.noff[t->T.nseg] + kRecPad;
+    ALLOC(B.rec_a, B.n_rec);
+    ALLOC(B.rec_b, B.n_rec);
+    ALLOC(B.partial, 6 * t->T.blk[t->T.nseg]);
+    ALLOC(B.roots, kMaxSeg);
+    ALLOC(B.dv, 3 * n);
+    ALLOC(B.err, 4);
+    ALLOC(B.labels, n);
+    ALLOC(B.maxlev, kMaxSeg);
+#undef ALLOC
+    HIPCHK(hipMemset(B.rec_a, 0, sizeof(float4) * B.n_rec));
+    HIPCHK(hipMemset(B.rec_b, 0, sizeof(int4) * B.n_rec));
+    HIPCHK(hipMemset(B.err, 0, 4 * sizeof(int)));
+    HIPCHK(hipMemset(B.dv, 0, 3 * sizeof(float) * n));
+    HIPCHK(hipHostMalloc((void **)&t->h_err, 4 * sizeof(int)));
+    return NBX_OK;
+}
+
+void free_bufs(nbx_tree *t)
+{
+    Bufs &B = t->B;
+    void *ptrs[] = {B.keys[0], B.keys[1], B.vals[0], B.vals[1], B.hist,    B.bodies, B.left,
+                    B.right,   B.par_int, B.par_leaf, B.plen,  B.flags,   B.cnt,    B.first,
+                    B.alloc,   B.bin_com, B.rec_a,   B.rec_b,  B.partial, B.roots,  B.dv,
+                    B.err,     B.labels,  B.maxlev};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (t->h_err) (void)hipHostFree(t->h_err);
+    B = Bufs{};
+    t->h_err = nullptr;
+}
+
+int check_tree_particles(nbx_tree *t, const nbx_particles *p)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    if (p->n != t->T.n_total)
+        return fail(NBX_E_ARG, "particle count %d != tree bodies %d", p->n, t->T.n_total);
+    return NBX_OK;
+}
+
+int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
+{
+    if (!out) return fail(NBX_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (nseg < 1 || nseg > kMaxSeg) return fail(NBX_E_ARG, "disk count %d not in [1,%d]", nseg,
+                                               kMaxSeg);
+    nbx_tree *t = new nbx_tree();
+    SegTable &T = t->T;
+    T.nseg = nseg;
+    long total = 0;
+    for (int s = 0; s < nseg; ++s) {
+        if (counts[s] <= 0) {
+            delete t;
+            return fail(NBX_E_ARG, "disk %d has %d bodies", s, counts[s]);
+        }
+        T.off[s] = (int)total;
+        total += counts[s];
+    }
+    if (total > (1L << 28)) {
+        delete t;
+        return fail(NBX_E_ARG, "too many bodies (%ld)", total);
+    }
+    T.off[nseg] = (int)total;
+    T.n_total = (int)total;
+    T.blk[0] = T.wave[0] = T.ioff[0] = T.noff[0] = 0;
+    for (int s = 0; s < nseg; ++s) {
+        const int n = counts[s];
+        T.blk[s + 1] = T.blk[s] + (n + kBlock - 1) / kBlock;
+        T.wave[s + 1] = T.wave[s] + (n + kWave - 1) / kWave;
+        T.ioff[s + 1] = T.ioff[s] + (n - 1);
+        T.noff[s + 1] = T.noff[s] + 2 * n + 1;
+    }
+    for (int s = nseg + 1; s <= kMaxSeg; ++s) {
+        T.off[s] = T.off[nseg];
+        T.blk[s] = T.blk[nseg];
+        T.wave[s] = T.wave[nseg];
+        T.ioff[s] = T.ioff[nseg];
+        T.noff[s] = T.noff[nseg];
+    }
+    hipError_t e = hipStreamCreate(&t->stream);
+    if (e != hipSuccess) {
+        delete t;
+        return fail(NBX_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    t->own_stream = true;
+    int rc = alloc_bufs(t);
+    if (rc != NBX_OK) {
+        std::string msg = g_err;
+        free_bufs(t);
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        g_err = msg;
+        return rc;
+    }
+    *out = t;
+    return NBX_OK;
+}
+
+DiskTable disk_table(const nbx_tree *t, const nbx_disk_params *dp, float sim_time, bool pert)
+{
+    DiskTable D;
+    memset(&D, 0, sizeof(D));
+    for (int s = 0; s < t->T.nseg; ++s) {
+        DiskDev &d = D.d[s];
+        if (!dp) continue;
+        d.r_coll = dp[s].r_coll;
+        if (pert && dp[s].perturber_enabled) {
+            // updatePerturber (Perturber.h:60-66), float arithmetic on the host
+            float phi = dp[s].pert_phase0 + dp[s].pert_omega * sim_time;
+            d.px = dp[s].pert_radius_au * cosf(phi);
+            d.pz = dp[s].pert_radius_au * sinf(phi);
+            d.py = 0.f;
+            d.pmass = dp[s].pert_mass_msun;
+        }
+    }
+    return D;
+}
+
+void ev_record(nbx_tree *t, int idx)
+{
+    if (t->timing) (void)hipEventRecord(t->ev[idx], t->stream);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *nbx_last_error(void) { return g_err.c_str(); }
+
+int nbx_particles_alloc(nbx_particles *p, int32_t n)
+{
+    if (!p || n <= 0) return fail(NBX_E_ARG, "nbx_particles_alloc: invalid arguments");
+    memset(p, 0, sizeof(*p));
+    float **f[10] = {&p->x, &p->y, &p->z, &p->vx, &p->vy, &p->vz, &p->ax, &p->ay, &p->az,
+                     &p->mass};
+    for (int k = 0; k < 10; ++k) {
+        HIPCHK(hipMalloc((void **)f[k], sizeof(float) * n));
+        HIPCHK(hipMemset(*f[k], 0, sizeof(float) * n));
+    }
+    HIPCHK(hipMalloc((void **)&p->id, sizeof(int32_t) * n));
+    HIPCHK(hipMemset(p->id, 0, sizeof(int32_t) * n));
+    p->n = n;
+    return NBX_OK;
+}
+
+int nbx_particles_free(nbx_particles *p)
+{
+    if (!p) return fail(NBX_E_ARG, "null particles");
+    void *f[11] = {p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay, p->az, p->mass, p->id};
+    for (void *q : f)
+        if (q) (void)hipFree(q);
+    memset(p, 0, sizeof(*p));
+    return NBX_OK;
+}
+
+static int copy_particles(const nbx_particles *src, nbx_particles *dst, hipMemcpyKind kind,
+                          const char *what)
+{
+    if (!src || !dst) return fail(NBX_E_ARG, "%s: null particles", what);
+    if (src->n != dst->n)
+        return fail(NBX_E_ARG, "%s: source n=%d != destination n=%d", what, src->n, dst->n);
+    const size_t fb = sizeof(float) * src->n;
+    const float *s[10] = {src->x, src->y, src->z, src->vx, src->vy, src->vz, src->ax, src->ay,
+                          src->az, src->mass};
+    float *d[10] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz, dst->ax, dst->ay,
+                    dst->az, dst->mass};
+    for (int k = 0; k < 10; ++k) HIPCHK(hipMemcpy(d[k], s[k], fb, kind));
+    if (src->id && dst->id) HIPCHK(hipMemcpy(dst->id, src->id, sizeof(int32_t) * src->n, kind));
+    return NBX_OK;
+}
+
+int nbx_copy_to_device(const nbx_particles *host, nbx_particles *dev)
+{
+    return copy_particles(host, dev, hipMemcpyHostToDevice, "copyToGPU");
+}
+
+int nbx_copy_to_host(const nbx_particles *dev, nbx_particles *host)
+{
+    return copy_particles(dev, host, hipMemcpyDeviceToHost, "copyToCPU");
+}
+
+int nbx_tree_create(nbx_tree **t, int32_t n_bodies)
+{
+    return create_tree(t, 1, &n_bodies);
+}
+
+int nbx_tree_create_batch(nbx_tree **t, int32_t n_seg, const int32_t *seg_counts)
+{
+    if (!seg_counts) return fail(NBX_E_ARG, "null disk counts");
+    return create_tree(t, n_seg, seg_counts);
+}
+
+int nbx_tree_destroy(nbx_tree *t)
+{
+    if (!t) return NBX_OK;
+    (void)hipStreamSynchronize(t->stream);
+    for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+    free_bufs(t);
+    if (t->own_stream) (void)hipStreamDestroy(t->stream);
+    delete t;
+    return NBX_OK;
+}
+
+int nbx_tree_reset(nbx_tree *t)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    launch_fill_float(t->B.dv, 0.f, 3 * t->T.n_total, t->stream);
+    launch_fill_int(t->B.err, 0, 4, t->stream);
+    t->stage = kNone;
+    return finish(t);
+}
+
+int nbx_tree_set_stream(nbx_tree *t, void *stream)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    HIPCHK(hipStreamSynchronize(t->stream));
+    if (stream) {
+        t->stream = (hipStream_t)stream;
+    } else if (!t->own_stream) {
+        HIPCHK(hipStreamCreate(&t->stream));
+        t->own_stream = true;
+    }
+    return NBX_OK;
+}
+
+int nbx_tree_set_async(nbx_tree *t, int async)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    t->async = async != 0;
+    return NBX_OK;
+}
+
+int nbx_tree_sync(nbx_tree *t)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    const bool a = t->async;
+    t->async = false;
+    int rc = finish(t);
+    t->async = a;
+    return rc;
+}
+
+int nbx_bbox(nbx_tree *t, const nbx_particles *p)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
+    launch_root(t->T, t->B, t->stream);
+    t->stage = kBBox;
+    return finish(t);
+}
+
+int nbx_build(nbx_tree *t, const nbx_particles *p)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (t->stage < kBBox) return fail(NBX_E_STATE, "launchBuildKernel before launchBBoxKernel");
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    t->stage = kBuilt;
+    return finish(t);
+}
+
+int nbx_summarize(nbx_tree *t)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (t->stage < kBuilt) return fail(NBX_E_STATE, "launchSummarizeKernel before build");
+    launch_summarize(t->T, t->B, t->stream);
+    t->stage = kSummarized;
+    return finish(t);
+}
+
+int nbx_sort(nbx_tree *t)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (t->stage < kBuilt) return fail(NBX_E_STATE, "launchSortKernel before build");
+    return finish(t);
+}
+
+int nbx_sort_idx(nbx_tree *t, int32_t *dev_out)
+{
+    if (!t || !dev_out) return fail(NBX_E_ARG, "null argument");
+    if (t->stage < kBuilt) return fail(NBX_E_STATE, "sort_idx before build");
+    HIPCHK(hipMemcpyAsync(dev_out, t->B.vals[0], sizeof(int32_t) * t->T.n_total,
+                          hipMemcpyDeviceToDevice, t->stream));
+    return finish(t);
+}
+
+int nbx_sorted_keys(nbx_tree *t, uint64_t *dev_out)
+{
+    if (!t || !dev_out) return fail(NBX_E_ARG, "null argument");
+    if (t->stage < kBuilt) return fail(NBX_E_STATE, "keys before build");
+    HIPCHK(hipMemcpyAsync(dev_out, t->B.keys[0], sizeof(uint64_t) * t->T.n_total,
+                          hipMemcpyDeviceToDevice, t->stream));
+    return finish(t);
+}
+
+int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, float star_mass,
+                 float *self_ax, float *self_ay, float *self_az)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "launchForceKernel before summarize");
+    if (!(theta > 0.f)) return fail(NBX_E_ARG, "theta must be positive");
+    WalkArgs W{theta * theta, softening * softening, softening, star_mass, p->ax, p->ay,
+               p->az, self_ax, self_ay, self_az};
+    DiskTable D = disk_table(t, nullptr, 0.f, false);
+    launch_walk(t->T, D, W, t->B, t->stream);
+    return finish(t);
+}
+
+int nbx_force(nbx_tree *t, nbx_particles *p, float theta, float softening, float star_mass)
+{
+    return nbx_force_ex(t, p, theta, softening, star_mass, nullptr, nullptr, nullptr);
+}
+
+int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n_open)
+{
+    if (!t || !dev_n_acc || !dev_n_open) return fail(NBX_E_ARG, "null argument");
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk counts before summarize");
+    launch_walk_counts(t->T, theta * theta, dev_n_acc, dev_n_open, t->B, t->stream);
+    return finish(t);
+}
+
+int nbx_collide(nbx_tree *t, nbx_particles *p, float r_coll, float restitution)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "launchCollisionKernel before summarize");
+    DiskTable D;
+    memset(&D, 0, sizeof(D));
+    for (int s = 0; s < t->T.nseg; ++s) D.d[s].r_coll = r_coll;
+    launch_collide(t->T, D, p->vx, p->vy, p->vz, restitution, t->B, t->stream);
+    // pass 2 (BHTree.cu:676-686): v += dv
+    launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az, t->B.dv, p->x, p->z, 0.f, 0, 1.f,
+                     0.f, p->n, t->stream);
+    return finish(t);
+}
+
+int nbx_perturber_force(nbx_particles *p, float px, float py, float pz, float pmass,
+                        float softening)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    launch_perturber(p->ax, p->ay, p->az, p->x, p->y, p->z, px, py, pz, pmass, softening, p->n,
+                     nullptr);
+    return finish_plain();
+}
+
+int nbx_half_kick(nbx_particles *p, float dt_half)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    launch_half_kick(p->vx, p->vy, p->vz, p->ax, p->ay, p->az, dt_half, p->n, nullptr);
+    return finish_plain();
+}
+
+int nbx_drift(nbx_particles *p, float dt)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    launch_drift(p->x, p->y, p->z, p->vx, p->vy, p->vz, dt, p->n, nullptr);
+    return finish_plain();
+}
+
+int nbx_reset_acceleration(nbx_particles *p)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    launch_reset_acc(p->ax, p->ay, p->az, p->n, nullptr);
+    return finish_plain();
+}
+
+int nbx_gas_drag(nbx_particles *p, float drag_rate, float dt, float star_mass)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
+    if (drag_rate <= 0.0f) return NBX_OK;  // Integrator.cu:102
+    const float decay = expf(-drag_rate * dt);
+    launch_gas_drag(p->x, p->z, p->vx, p->vy, p->vz, decay, star_mass, p->n, nullptr);
+    return finish_plain();
+}
+
+int nbx_fof(nbx_tree *t, const nbx_particles *p, float link_length, int32_t *dev_labels)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!dev_labels) return fail(NBX_E_ARG, "null labels");
+    launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
+    launch_root(t->T, t->B, t->stream);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_summarize(t->T, t->B, t->stream);
+    t->stage = kSummarized;
+    launch_fof(t->T, link_length, dev_labels, t->B, t->stream);
+    return finish(t);
+}
+
+int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                    const nbx_disk_params *dp)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
+    launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
+    launch_root(t->T, t->B, t->stream);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_summarize(t->T, t->B, t->stream);
+    t->stage = kSummarized;
+    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
+               sp->star_mass, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+    DiskTable D = disk_table(t, dp, 0.0f, true);
+    launch_walk(t->T, D, W, t->B, t->stream);
+    return finish(t);
+}
+
+int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+             const nbx_disk_params *dp, int32_t step0, int32_t n_steps)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
+    if (n_steps < 0 || step0 < 0) return fail(NBX_E_ARG, "negative step range");
+    const int nev = 6 * n_steps;
+    if (t->timing && (int)t->ev.size() < nev) {
+        const size_t old = t->ev.size();
+        t->ev.resize(nev);
+        for (size_t q = old; q < t->ev.size(); ++q) HIPCHK(hipEventCreate(&t->ev[q]));
+    }
+    const float dt = sp->dt, dth = sp->dt * 0.5f;
+    const int has_drag = sp->gas_drag_rate > 0.0f;
+    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;
+    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
+               sp->star_mass, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+    for (int k = 0; k < n_steps; ++k) {
+        const int step = step0 + k;
+        const float sim_time = (float)(step + 1) * dt;  // main.cu:110
+        const DiskTable D = disk_table(t, dp, sim_time, true);
+        const int e0 = 6 * k;
+        ev_record(t, e0 + 0);
+        launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
+                               p->az, dth, dt, t->B, t->stream);
+        ev_record(t, e0 + 1);
+        launch_root(t->T, t->B, t->stream);
+        launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+        launch_summarize(t->T, t->B, t->stream);
+        ev_record(t, e0 + 2);
+        launch_walk(t->T, D, W, t->B, t->stream);
+        ev_record(t, e0 + 3);
+        if (sp->collisions)
+            launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
+        ev_record(t, e0 + 4);
+        launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
+                         sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag, decay,
+                         sp->star_mass, p->n, t->stream);
+        ev_record(t, e0 + 5);
+    }
+    if (t->timing && n_steps > 0) {
+        float acc_ms[5] = {0, 0, 0, 0, 0};
+        HIPCHK(hipEventSynchronize(t->ev[nev - 1]));
+        for (int k = 0; k < n_steps; ++k)
+            for (int q = 0; q < 5; ++q) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, t->ev[6 * k + q], t->ev[6 * k + q + 1]));
+                acc_ms[q] += ms;
+            }
+        for (int q = 0; q < 5; ++q) t->phase_ms[q] = acc_ms[q] / n_steps;
+        t->phase_n = n_steps;
+    }
+    t->stage = kSummarized;
+    return finish(t);
+}
+
+int nbx_tree_stats(nbx_tree *t, int32_t *n_nodes, int32_t *max_level)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "statistics before summarize");
+    std::vector<int> alloc(kMaxSeg), lev(kMaxSeg);
+    HIPCHK(hipStreamSynchronize(t->stream));
+    HIPCHK(hipMemcpy(alloc.data(), t->B.alloc, sizeof(int) * kMaxSeg, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lev.data(), t->B.maxlev, sizeof(int) * kMaxSeg, hipMemcpyDeviceToHost));
+    for (int s = 0; s < t->T.nseg; ++s) {
+        if (n_nodes) n_nodes[s] = alloc[s] + 1;  // records incl. the root
+        if (max_level) max_level[s] = lev[s];
+    }
+    return NBX_OK;
+}
+
+int nbx_tree_set_timing(nbx_tree *t, int on)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    t->timing = on != 0;
+    return NBX_OK;
+}
+
+int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms)
+{
+    if (!t || !phase || !ms) return fail(NBX_E_ARG, "null argument");
+    for (int q = 0; q < 5; ++q)
+        if (!strcmp(phase, kPhases[q])) {
+            *ms = t->phase_ms[q];
+            return NBX_OK;
+        }
+    return fail(NBX_E_ARG, "unknown phase '%s'", phase);
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// nbx_internal.hpp -- shared device/host definitions of the gfx950 disk step.
+//
+// Data layout in HBM (one nbx_tree, all disks of a batch back to back):
+//   state      caller's ParticleData SoA in ORIGINAL body order (never
+//              permuted, so id == index as in the reference, BHTree.cu:541)
+//   keys/perm  63-bit Morton keys and body index, sorted (double-buffered
+//              for the radix passes)
+//   bodies     float4 {x,y,z,m} gathered in sorted order (walk leaves)
+//   bin_*      binary radix tree over the sorted keys (n-1 nodes per disk)
+//   rec_a/b    walk records of the compressed octree: float4 {com, mass}
+//              and int4 {h, h^2, first child record, child count}; the
+//              children of a node are contiguous, the root record of disk s
+//              sits at node_off[s].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbx {
+
+constexpr int kMaxSeg = 16;      // disks per batched tree
+constexpr int kLevels = 21;      // octant digits per key (63 bits)
+constexpr int kBlock = 256;      // threads per block for streaming kernels
+constexpr int kWave = 64;        // gfx950 wavefront
+constexpr int kStackCap = 384;   // walk stack entries per wave (LDS)
+constexpr int kRecPad = 16;      // records of padding after the node array
+
+// Per-disk offsets, passed by value to every kernel.
+struct SegTable {
+    int nseg;
+    int n_total;                 // bodies over all disks
+    int off[kMaxSeg + 1];        // body offset of disk s (prefix of counts)
+    int blk[kMaxSeg + 1];        // kBlock-sized block offset of disk s
+    int wave[kMaxSeg + 1];       // wave (64 targets) offset of disk s
+    int ioff[kMaxSeg + 1];       // binary internal-node offset (n_s - 1 each)
+    int noff[kMaxSeg + 1];       // walk-record offset (2 n_s + 1 each)
+};
+
+__host__ __device__ inline int seg_of_block(const SegTable &T, int b)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && b >= T.blk[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_body(const SegTable &T, int i)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && i >= T.off[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_wave(const SegTable &T, int w)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && w >= T.wave[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_internal(const SegTable &T, int j)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && j >= T.ioff[s + 1]) ++s;
+    return s;
+}
+
+// Per-disk physical parameters of one step.
+struct DiskDev {
+    float r_coll;
+    float pmass;                 // perturber mass in M_sun (0 = disabled)
+    float px, py, pz;            // perturber position at the step's sim_time
+};
+
+struct DiskTable {
+    DiskDev d[kMaxSeg];
+};
+
+// Root cube per disk: {cx, cy, cz, half}.
+struct Root {
+    float c[4];
+};
+
+// Error flags written by kernels (host reads after the step).
+enum ErrBits : int {
+    kErrStack = 1,       // walk stack overflow
+    kErrTree = 2,        // inconsistent tree (should not happen)
+};
+
+}  // namespace nbx

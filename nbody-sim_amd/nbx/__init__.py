@@ -1,0 +1,503 @@
+"""Python side of the MI355X Barnes-Hut disk step.
+
+A thin ctypes binding of the C ABI in include/nbx.h (libnbx.so, built from
+nbody-sim_amd/csrc by :func:`build`).  Device memory comes from PyTorch-ROCm
+tensors; nothing here computes physics on the host.  The function names
+mirror the reference's call surface (BHTree.cuh:38-50, Integrator.cuh:9-22,
+Perturber.h:39-75, Aggregate.h:17-18, ParticleData.h:35-123) so code written
+against the reference reads the same.
+
+The HIP library is mandatory: if libnbx.so is missing or cannot be loaded,
+every entry point raises RuntimeError (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PKG = os.path.dirname(_HERE)
+_REPO = os.path.dirname(_PKG)
+CSRC = os.path.join(_PKG, "csrc")
+LIB_PATH = os.path.join(_HERE, "libnbx.so")
+HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_abi.hip"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # keep hand-written float code on single-lane VALU ops (no v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+MJUP_TO_MSUN = 9.5479e-4  # Perturber.h:25
+
+
+def build(verbose=False):
+    """Compile libnbx.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["hipcc", *HIPCC_FLAGS, "-o", tmp, *srcs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print("built", LIB_PATH)
+    return LIB_PATH
+
+
+class nbx_particles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+                ("ax", C.c_void_p), ("ay", C.c_void_p), ("az", C.c_void_p),
+                ("mass", C.c_void_p), ("id", C.c_void_p), ("n", C.c_int32)]
+
+
+class nbx_step_params(C.Structure):
+    _fields_ = [("dt", C.c_float), ("theta", C.c_float), ("softening", C.c_float),
+                ("star_mass", C.c_float), ("restitution", C.c_float),
+                ("gas_drag_rate", C.c_float), ("collisions", C.c_int32),
+                ("use_graph", C.c_int32)]
+
+
+class nbx_disk_params(C.Structure):
+    _fields_ = [("r_coll", C.c_float), ("perturber_enabled", C.c_int32),
+                ("pert_mass_msun", C.c_float), ("pert_radius_au", C.c_float),
+                ("pert_phase0", C.c_float), ("pert_omega", C.c_float)]
+
+
+# every symbol include/nbx.h declares (checked by the CPU test suite)
+EXPORTS = [
+    "nbx_last_error", "nbx_particles_alloc", "nbx_particles_free", "nbx_copy_to_device",
+    "nbx_copy_to_host", "nbx_tree_create", "nbx_tree_create_batch", "nbx_tree_destroy",
+    "nbx_tree_reset", "nbx_tree_set_stream", "nbx_tree_set_async", "nbx_tree_sync", "nbx_bbox",
+    "nbx_build", "nbx_summarize", "nbx_sort", "nbx_sort_idx", "nbx_sorted_keys", "nbx_force",
+    "nbx_force_ex", "nbx_collide", "nbx_perturber_force", "nbx_half_kick", "nbx_drift",
+    "nbx_reset_acceleration", "nbx_gas_drag", "nbx_fof", "nbx_init_forces", "nbx_step",
+    "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_walk_counts",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libnbx.so (raises RuntimeError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP extension not built: {LIB_PATH} missing (run nbx.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, f32 = C.c_void_p, C.c_int32, C.c_float
+        P = C.POINTER(nbx_particles)
+        L.nbx_last_error.restype = C.c_char_p
+        sig = {
+            "nbx_particles_alloc": [P, i32], "nbx_particles_free": [P],
+            "nbx_copy_to_device": [P, P], "nbx_copy_to_host": [P, P],
+            "nbx_tree_create": [C.POINTER(vp), i32],
+            "nbx_tree_create_batch": [C.POINTER(vp), i32, C.POINTER(i32)],
+            "nbx_tree_destroy": [vp], "nbx_tree_reset": [vp],
+            "nbx_tree_set_stream": [vp, vp], "nbx_tree_set_async": [vp, i32],
+            "nbx_tree_sync": [vp], "nbx_bbox": [vp, P], "nbx_build": [vp, P],
+            "nbx_summarize": [vp], "nbx_sort": [vp], "nbx_sort_idx": [vp, vp],
+            "nbx_sorted_keys": [vp, vp], "nbx_force": [vp, P, f32, f32, f32],
+            "nbx_force_ex": [vp, P, f32, f32, f32, vp, vp, vp],
+            "nbx_collide": [vp, P, f32, f32], "nbx_perturber_force": [P, f32, f32, f32, f32, f32],
+            "nbx_half_kick": [P, f32], "nbx_drift": [P, f32], "nbx_reset_acceleration": [P],
+            "nbx_gas_drag": [P, f32, f32, f32], "nbx_fof": [vp, P, f32, vp],
+            "nbx_init_forces": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params)],
+            "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
+            "nbx_tree_stats": [vp, vp, vp], "nbx_tree_set_timing": [vp, i32],
+            "nbx_phase_ms": [vp, C.c_char_p, C.POINTER(f32)],
+            "nbx_walk_counts": [vp, f32, vp, vp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().nbx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible: the nbx step runs only on the HIP device")
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# ParticleData (ParticleData.h:19-123)
+# ---------------------------------------------------------------------------
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "mass")
+
+
+@dataclass
+class ParticleData:
+    """SoA particle state in HBM (torch tensors on the HIP device)."""
+    n: int
+    t: dict = field(default_factory=dict)
+
+    def c(self):
+        s = nbx_particles()
+        for k in FIELDS:
+            setattr(s, k, self.t[k].data_ptr())
+        s.id = self.t["id"].data_ptr()
+        s.n = self.n
+        return s
+
+    def __getattr__(self, k):
+        if k in FIELDS or k == "id":
+            return self.t[k]
+        raise AttributeError(k)
+
+    def numpy(self):
+        _torch().cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in self.t.items()}
+
+
+def allocateParticlesGPU(n):
+    torch = _torch()
+    if n <= 0:
+        raise RuntimeError("allocateParticlesGPU: n must be positive")
+    p = ParticleData(n)
+    for k in FIELDS:
+        p.t[k] = torch.zeros(n, dtype=torch.float32, device="cuda")
+    p.t["id"] = torch.arange(n, dtype=torch.int32, device="cuda")
+    return p
+
+
+def copyToGPU(cpu, gpu):
+    """Host dict of numpy arrays -> device ParticleData (ParticleData.h:90-105)."""
+    torch = _torch()
+    n = len(cpu["x"])
+    if n != gpu.n:
+        raise RuntimeError(f"copyToGPU: CPU n={n} != GPU n={gpu.n}")
+    for k in FIELDS:
+        gpu.t[k].copy_(torch.from_numpy(np.ascontiguousarray(cpu[k], np.float32)))
+    if "id" in cpu:
+        gpu.t["id"].copy_(torch.from_numpy(np.ascontiguousarray(cpu["id"], np.int32)))
+    torch.cuda.synchronize()
+
+
+def copyToCPU(gpu):
+    """Device ParticleData -> host dict of numpy arrays (ParticleData.h:108-123)."""
+    return gpu.numpy()
+
+
+def to_gpu(cpu):
+    p = allocateParticlesGPU(len(cpu["x"]))
+    copyToGPU(cpu, p)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# OctreeData and the launchers (BHTree.cuh:38-50)
+# ---------------------------------------------------------------------------
+class OctreeData:
+    """Opaque nbx_tree handle (tree scratch, stream) for one or more disks."""
+
+    def __init__(self, n_bodies=None, seg_counts=None):
+        h = C.c_void_p()
+        if seg_counts is not None:
+            arr = (C.c_int32 * len(seg_counts))(*seg_counts)
+            _check(lib().nbx_tree_create_batch(C.byref(h), len(seg_counts), arr),
+                   "allocateOctree(batch)")
+            self.seg_counts = list(seg_counts)
+        else:
+            _check(lib().nbx_tree_create(C.byref(h), int(n_bodies)), "allocateOctree")
+            self.seg_counts = [int(n_bodies)]
+        self.h = h
+        self.n_bodies = sum(self.seg_counts)
+
+    def free(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().nbx_tree_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def sort_idx(self):
+        torch = _torch()
+        out = torch.empty(self.n_bodies, dtype=torch.int32, device="cuda")
+        _check(lib().nbx_sort_idx(self.h, out.data_ptr()), "sort_idx")
+        return out
+
+    def sorted_keys(self):
+        torch = _torch()
+        out = torch.empty(self.n_bodies, dtype=torch.int64, device="cuda")
+        _check(lib().nbx_sorted_keys(self.h, out.data_ptr()), "sorted_keys")
+        return out
+
+    def stats(self):
+        ns = len(self.seg_counts)
+        a = (C.c_int32 * ns)()
+        b = (C.c_int32 * ns)()
+        _check(lib().nbx_tree_stats(self.h, a, b), "tree_stats")
+        return list(a), list(b)
+
+
+def allocateOctree(n_bodies):
+    return OctreeData(n_bodies)
+
+
+def allocateOctreeBatch(seg_counts):
+    return OctreeData(seg_counts=seg_counts)
+
+
+def freeOctree(t):
+    t.free()
+
+
+def resetOctree(t):
+    _check(lib().nbx_tree_reset(t.h), "resetOctree")
+
+
+def launchBBoxKernel(t, p):
+    _check(lib().nbx_bbox(t.h, C.byref(p.c())), "launchBBoxKernel")
+
+
+def launchBuildKernel(t, p):
+    _check(lib().nbx_build(t.h, C.byref(p.c())), "launchBuildKernel")
+
+
+def launchSummarizeKernel(t):
+    _check(lib().nbx_summarize(t.h), "launchSummarizeKernel")
+
+
+def launchSortKernel(t):
+    _check(lib().nbx_sort(t.h), "launchSortKernel")
+
+
+def launchForceKernel(t, p, theta, softening, star_mass, self_acc=None):
+    """a = star + tree self-gravity; self_acc (3 tensors) receives the tree part."""
+    if self_acc is None:
+        _check(lib().nbx_force(t.h, C.byref(p.c()), theta, softening, star_mass),
+               "launchForceKernel")
+    else:
+        sx, sy, sz = self_acc
+        _check(lib().nbx_force_ex(t.h, C.byref(p.c()), theta, softening, star_mass,
+                                  sx.data_ptr(), sy.data_ptr(), sz.data_ptr()),
+               "launchForceKernel")
+
+
+def launchCollisionKernel(t, p, r_coll, restitution):
+    _check(lib().nbx_collide(t.h, C.byref(p.c()), r_coll, restitution), "launchCollisionKernel")
+
+
+def launchHalfKickKernel(p, dt_half):
+    _check(lib().nbx_half_kick(C.byref(p.c()), dt_half), "launchHalfKickKernel")
+
+
+def launchDriftKernel(p, dt):
+    _check(lib().nbx_drift(C.byref(p.c()), dt), "launchDriftKernel")
+
+
+def launchResetAccelerationKernel(p):
+    _check(lib().nbx_reset_acceleration(C.byref(p.c())), "launchResetAccelerationKernel")
+
+
+def launchGasDragKernel(p, drag_rate, dt, star_mass):
+    _check(lib().nbx_gas_drag(C.byref(p.c()), drag_rate, dt, star_mass), "launchGasDragKernel")
+
+
+def launchPerturberForceKernel(p, px, py, pz, pmass, softening):
+    _check(lib().nbx_perturber_force(C.byref(p.c()), px, py, pz, pmass, softening),
+           "launchPerturberForceKernel")
+
+
+# ---------------------------------------------------------------------------
+# Perturber (Perturber.h:27-66): host-side float arithmetic
+# ---------------------------------------------------------------------------
+@dataclass
+class Perturber:
+    enabled: bool = False
+    r: float = 0.0
+    mass_msun: float = 0.0
+    phase0: float = 0.0
+    omega: float = 0.0
+    x: float = 0.0
+    y: float = 0.0
+    z: float = 0.0
+
+
+def _f(v):
+    return float(np.float32(v))
+
+
+def makePerturber(enabled, mass_mjup, radius_au, phase0=0.0, star_mass=1.0):
+    """makePerturber (Perturber.h:39-57) in float32 arithmetic."""
+    p = Perturber(enabled=bool(enabled))
+    if not p.enabled:
+        return p
+    f = np.float32
+    p.r = _f(radius_au)
+    p.mass_msun = _f(f(mass_mjup) * f(MJUP_TO_MSUN))
+    p.phase0 = _f(phase0)
+    r = f(p.r)
+    p.omega = _f(np.sqrt(f(star_mass) / (r * r * r)))
+    p.x = _f(r * np.cos(f(p.phase0)))
+    p.z = _f(r * np.sin(f(p.phase0)))
+    p.y = 0.0
+    return p
+
+
+def updatePerturber(p, sim_time):
+    """updatePerturber (Perturber.h:60-66) in float32 arithmetic."""
+    if not p.enabled:
+        return p
+    f = np.float32
+    phi = f(p.phase0) + f(p.omega) * f(sim_time)
+    p.x = _f(f(p.r) * np.cos(phi))
+    p.z = _f(f(p.r) * np.sin(phi))
+    p.y = 0.0
+    return p
+
+
+def particlePhysicalRadius(particle_mass_msun):
+    """particlePhysicalRadius (DiskInit.cpp:10-16) in float32 arithmetic."""
+    f = np.float32
+    mass_kg = f(particle_mass_msun) * f(1.989e30)
+    vol = mass_kg / f(3000.0)
+    r_m = np.cbrt(f(3.0) * vol / (f(4.0) * f(3.14159265358979)))
+    return _f(f(r_m) / f(1.496e11))
+
+
+# ---------------------------------------------------------------------------
+# FoF (Aggregate.h, Aggregate.cpp:26-76)
+# ---------------------------------------------------------------------------
+@dataclass
+class Aggregate:
+    id: int
+    n_particles: int
+    mass_msun: float
+    cx: float
+    cy: float
+    cz: float
+    vx: float
+    vy: float
+    vz: float
+    dist_center: float
+    particle_ids: list
+
+
+def walk_counts(t, p, theta):
+    """Mean accepted / opened records per body of the theta walk on the tree
+    last summarized from p."""
+    torch = _torch()
+    a = torch.zeros(p.n, dtype=torch.int32, device="cuda")
+    o = torch.zeros(p.n, dtype=torch.int32, device="cuda")
+    _check(lib().nbx_walk_counts(t.h, theta, a.data_ptr(), o.data_ptr()), "walk_counts")
+    return float(a.double().mean()), float(o.double().mean())
+
+
+def fof_into(t, p, link_length, labels):
+    """FoF labels into an existing int32 device tensor (no allocation)."""
+    _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, labels.data_ptr()), "fof")
+
+
+def fof_labels(t, p, link_length):
+    """Device FoF partition: labels[i] = smallest index of i's group."""
+    torch = _torch()
+    out = torch.empty(p.n, dtype=torch.int32, device="cuda")
+    _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, out.data_ptr()), "fof")
+    return out
+
+
+def aggregates_from_labels(labels, host, particle_mass_msun):
+    """Groups of >= 2 bodies with the float32 sums of Aggregate.cpp:58-69
+    (members in ascending index), ordered by smallest member index."""
+    labels = np.asarray(labels)
+    order = np.argsort(labels, kind="stable")
+    lab_sorted = labels[order]
+    starts = np.flatnonzero(np.r_[True, lab_sorted[1:] != lab_sorted[:-1]])
+    ends = np.r_[starts[1:], len(labels)]
+    f = np.float32
+    out = []
+    for s, e in zip(starts, ends):
+        if e - s < 2:
+            continue
+        members = np.sort(order[s:e])
+        acc = [f(0)] * 6
+        for idx in members:
+            for k, key in enumerate(("x", "y", "z", "vx", "vy", "vz")):
+                acc[k] = f(acc[k] + f(host[key][idx]))
+        inv = f(1.0) / f(len(members))
+        c = [f(a * inv) for a in acc]
+        out.append(Aggregate(
+            id=len(out), n_particles=int(len(members)),
+            mass_msun=_f(f(len(members)) * f(particle_mass_msun)),
+            cx=_f(c[0]), cy=_f(c[1]), cz=_f(c[2]), vx=_f(c[3]), vy=_f(c[4]), vz=_f(c[5]),
+            dist_center=_f(np.sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2])),
+            particle_ids=[int(m) for m in members]))
+    return out
+
+
+def detectAggregates(t, p, link_length, particle_mass_msun):
+    """detectAggregates (Aggregate.h:17-18) on device data: partition on the GPU,
+    per-group float32 sums on the host as the reference does."""
+    labels = fof_labels(t, p, link_length).cpu().numpy()
+    return aggregates_from_labels(labels, p.numpy(), particle_mass_msun)
+
+
+# ---------------------------------------------------------------------------
+# Fused step (main.cu:85-140)
+# ---------------------------------------------------------------------------
+def step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                gas_drag_rate=0.0, collisions=True, use_graph=False):
+    return nbx_step_params(dt, theta, softening, star_mass, restitution, gas_drag_rate,
+                           int(collisions), int(use_graph))
+
+
+def disk_params(r_coll=0.0, perturber=None):
+    """One nbx_disk_params from r_coll and a Perturber (or None)."""
+    if perturber is None or not perturber.enabled:
+        return nbx_disk_params(r_coll, 0, 0.0, 0.0, 0.0, 0.0)
+    return nbx_disk_params(r_coll, 1, perturber.mass_msun, perturber.r, perturber.phase0,
+                           perturber.omega)
+
+
+def _disk_array(dps):
+    if isinstance(dps, nbx_disk_params):
+        dps = [dps]
+    return (nbx_disk_params * len(dps))(*dps)
+
+
+def init_forces(t, p, sp, dps):
+    _check(lib().nbx_init_forces(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps)),
+           "init_forces")
+
+
+def step(t, p, sp, dps, step0, n_steps):
+    _check(lib().nbx_step(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps), step0, n_steps),
+           "step")
+
+
+def set_async(t, on):
+    _check(lib().nbx_tree_set_async(t.h, int(on)), "set_async")
+
+
+def sync(t):
+    _check(lib().nbx_tree_sync(t.h), "sync")
+
+
+def set_timing(t, on):
+    _check(lib().nbx_tree_set_timing(t.h, int(on)), "set_timing")
+
+
+def phase_ms(t, phase):
+    v = C.c_float()
+    _check(lib().nbx_phase_ms(t.h, phase.encode(), C.byref(v)), "phase_ms")
+    return v.value

@@ -1,0 +1,380 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact where the arithmetic is integer or written without contraction on
+both sides (keys, order, FoF labels, KDK / drag / perturber, single-partner
+collision impulses); tolerance-based for the tree forces, with the
+tolerance stated in each test.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import clustered, disk, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _build(nbx, p, tree=None):
+    t = tree or nbx.allocateOctree(p.n)
+    nbx.launchBBoxKernel(t, p)
+    nbx.launchBuildKernel(t, p)
+    nbx.launchSummarizeKernel(t)
+    nbx.launchSortKernel(t)
+    return t
+
+
+# ---------------------------------------------------------------------------
+# Integrator / perturber: bit-exact (Integrator.cu, Perturber.cu)
+# ---------------------------------------------------------------------------
+def test_kdk_drag_perturber_bit_exact(gpu):
+    nbx = gpu
+    d = disk(4096, seed=3)
+    rng = np.random.default_rng(5)
+    for k in ("ax", "ay", "az"):
+        d[k] = rng.normal(0, 0.3, 4096).astype(np.float32)
+    p = nbx.to_gpu(d)
+    nbx.launchHalfKickKernel(p, np.float32(0.005))
+    v = O.half_kick([d["vx"], d["vy"], d["vz"]], [d["ax"], d["ay"], d["az"]], 0.005)
+    g = p.numpy()
+    for k, ref in zip(("vx", "vy", "vz"), v):
+        np.testing.assert_array_equal(g[k], ref)
+    nbx.launchDriftKernel(p, 0.01)
+    x = O.drift([d["x"], d["y"], d["z"]], v, 0.01)
+    g = p.numpy()
+    for k, ref in zip(("x", "y", "z"), x):
+        np.testing.assert_array_equal(g[k], ref)
+    nbx.launchGasDragKernel(p, 1e-3, 0.01, 1.0)
+    v = O.gas_drag(x, v, 1e-3, 0.01, 1.0)
+    g = p.numpy()
+    for k, ref in zip(("vx", "vy", "vz"), v):
+        np.testing.assert_array_equal(g[k], ref)
+    nbx.launchPerturberForceKernel(p, 5.2, 0.0, 0.3, 9.5479e-3, 1e-3)
+    a = O.perturber_force([d["ax"], d["ay"], d["az"]], x, (5.2, 0.0, 0.3), 9.5479e-3, 1e-3)
+    g = p.numpy()
+    for k, ref in zip(("ax", "ay", "az"), a):
+        np.testing.assert_array_equal(g[k], ref)
+    nbx.launchResetAccelerationKernel(p)
+    assert not p.ax.any() and not p.ay.any() and not p.az.any()
+
+
+def test_perturber_kernel_kats(gpu):
+    """test_perturber.cu:83-149: +x pull on an origin particle, and '+=' semantics."""
+    nbx = gpu
+    f = np.float32
+    zero = dict(x=np.zeros(1, f), y=np.zeros(1, f), z=np.zeros(1, f), vx=np.zeros(1, f),
+                vy=np.zeros(1, f), vz=np.zeros(1, f), ax=np.zeros(1, f), ay=np.zeros(1, f),
+                az=np.zeros(1, f), mass=np.ones(1, f))
+    p = nbx.to_gpu(zero)
+    nbx.launchPerturberForceKernel(p, 10.0, 0.0, 0.0, 1e-3, 0.01)
+    r2 = f(100.0) + f(0.01) * f(0.01)
+    expected = f(1e-3) / (r2 * np.sqrt(r2)) * f(10.0)
+    g = p.numpy()
+    assert abs(g["ax"][0] - expected) <= 1e-4 * abs(expected)
+    assert abs(g["ay"][0]) <= 1e-7 and abs(g["az"][0]) <= 1e-7
+    zero["ax"] = np.full(1, 1e-3, f)
+    p = nbx.to_gpu(zero)
+    nbx.launchPerturberForceKernel(p, 10.0, 0.0, 0.0, 1e-3, 0.01)
+    assert abs(p.numpy()["ax"][0] - (1e-3 + expected)) <= 1e-4 * (1e-3 + expected)
+
+
+# ---------------------------------------------------------------------------
+# Keys and order: bit-exact against the oracle and against the reference DFS
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n,seed", [(10, 0), (1000, 1), (4096, 2), (131072, 3)])
+def test_keys_and_order_bit_exact(gpu, n, seed):
+    nbx = gpu
+    d = disk(n, seed=seed)
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    keys = O.mt_keys(d["x"], d["y"], d["z"])
+    perm = O.mt_sort(keys)
+    g_perm = _np(t.sort_idx())
+    g_keys = _np(t.sorted_keys()).view(np.uint64)
+    np.testing.assert_array_equal(g_perm, perm)
+    np.testing.assert_array_equal(g_keys, keys[perm])
+    # permutation property of test_bhtree.cu:12-24
+    assert np.array_equal(np.sort(g_perm), np.arange(n))
+    if n <= 4096:
+        # the reference's DFS order (BHTree.cu:322-352) equals the Morton order
+        # when no two bodies share all 21 octant digits
+        ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"])
+        if len(np.unique(keys)) == n:
+            np.testing.assert_array_equal(g_perm, ref.sort_idx())
+
+
+def test_line_of_ten_permutation(gpu):
+    """test_bhtree.cu:29-59: 10 bodies on a line build and sort."""
+    nbx = gpu
+    f = np.float32
+    n = 10
+    d = {k: np.zeros(n, f) for k in nbx.FIELDS}
+    d["x"] = (np.arange(n) * 0.5).astype(f)
+    d["mass"][:] = 3.003e-7
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    perm = _np(t.sort_idx())
+    assert sorted(perm.tolist()) == list(range(n))
+    np.testing.assert_array_equal(perm, O.RefTree(d["x"], d["y"], d["z"], d["mass"]).sort_idx())
+
+
+# ---------------------------------------------------------------------------
+# Forces
+# ---------------------------------------------------------------------------
+def test_single_particle_star_force(gpu):
+    """test_bhtree.cu:104-134: one body at (1,0,0) feels a_x = -1 within 5 %."""
+    nbx = gpu
+    f = np.float32
+    d = {k: np.zeros(1, f) for k in nbx.FIELDS}
+    d["x"][0] = 1.0
+    d["mass"][0] = 1e-6
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    nbx.launchResetAccelerationKernel(p)
+    nbx.launchForceKernel(t, p, 0.5, 0.001, 1.0)
+    g = p.numpy()
+    assert abs(g["ax"][0] + 1.0) <= 0.05
+    assert abs(g["ay"][0]) <= 0.05 and abs(g["az"][0]) <= 0.05
+    # the star term is the reference expression evaluated exactly
+    ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"]).force(0.5, 0.001, 1.0)
+    assert g["ax"][0] == ref["a"][0, 0]
+
+
+@pytest.mark.parametrize("n,mass,seed", [(4096, 1e-5, 1), (131072, 3e-5, 2)])
+def test_force_vs_oracle(gpu, n, mass, seed):
+    """Tree self-gravity vs the oracle's walk of the same compressed octree
+    (median rel diff <= 2e-6, 99.9th pct <= 1e-3: COM rounding and the
+    rsqrt approximation), total acceleration vs the reference walk restated
+    on the canonical octree (max rel diff <= 2e-6), and the error against
+    the FP64 direct sum stated at theta = 0.5."""
+    import torch
+    nbx = gpu
+    d = disk(n, mass=mass, seed=seed)
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    sa = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0, self_acc=sa)
+    g_self = np.stack([_np(s) for s in sa], 1)
+    g = p.numpy()
+    g_tot = np.stack([g["ax"], g["ay"], g["az"]], 1)
+    mt = O.mt_force(d["x"], d["y"], d["z"], d["mass"], 0.5, 1e-3)
+    e = rel_err(g_self, mt["self_a"])
+    assert np.median(e) <= 2e-6, np.median(e)
+    assert np.percentile(e, 99.9) <= 1e-3, np.percentile(e, 99.9)
+    ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"]).force(0.5, 1e-3, 1.0)
+    et = rel_err(g_tot, ref["a"])
+    assert et.max() <= 2e-6, et.max()
+    # accuracy against FP64 direct sum on 1024 seeded targets (reference
+    # opening rule at theta=0.5, monopole): stated, loosely bounded
+    tg = np.sort(np.random.default_rng(0).choice(n, 1024, replace=False)).astype(np.int32)
+    ad = O.direct_f64(d["x"], d["y"], d["z"], d["mass"], 1e-3, targets=tg)
+    ed = rel_err(g_self[tg], ad)
+    assert np.median(ed) <= 0.1, np.median(ed)
+
+
+def test_force_two_bodies_exact_pair(gpu):
+    """Two bodies: leaves are always accepted, so the tree term is the exact
+    softened pair force."""
+    nbx = gpu
+    import torch
+    f = np.float32
+    d = {k: np.zeros(2, f) for k in nbx.FIELDS}
+    d["x"][:] = [1.0, 1.3]
+    d["z"][:] = [0.0, 0.4]
+    d["mass"][:] = [1e-6, 2e-6]
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    sa = [torch.zeros(2, dtype=torch.float32, device="cuda") for _ in range(3)]
+    nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0, self_acc=sa)
+    g = np.stack([_np(s) for s in sa], 1)
+    ad = O.direct_f64(d["x"], d["y"], d["z"], d["mass"], 1e-3)
+    assert rel_err(g, ad).max() <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Collisions
+# ---------------------------------------------------------------------------
+def test_collision_head_on_kat(gpu):
+    """test_collision.cu:7-48."""
+    nbx = gpu
+    f = np.float32
+    d = {k: np.zeros(2, f) for k in nbx.FIELDS}
+    d["x"][:] = [-0.001, 0.001]
+    d["vx"][:] = [0.1, -0.1]
+    d["mass"][:] = 3.003e-7
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    nbx.launchCollisionKernel(t, p, 0.005, 0.3)
+    g = p.numpy()
+    assert g["vx"][0] < 0 and g["vx"][1] > 0
+    assert abs(g["vx"][0] + g["vx"][1]) <= 1e-5
+    ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"])
+    vx, vy, vz, pairs = ref.collide(d["vx"], d["vy"], d["vz"], 0.005, 0.3, exact=True)
+    np.testing.assert_array_equal(g["vx"], vx)
+
+
+def test_collision_vs_oracle_exact_pairs(gpu):
+    """Clustered bodies: same pair set and impulses as the oracle's exact
+    collision pass (abs diff <= 1e-6 |v|: summation order of multi-partner
+    deltas)."""
+    nbx = gpu
+    d = clustered(300, 4, 2e-3, seed=7)
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    rc = 3e-3
+    nbx.launchCollisionKernel(t, p, rc, 0.2)
+    g = p.numpy()
+    ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"])
+    vx, vy, vz, pairs = ref.collide(d["vx"], d["vy"], d["vz"], rc, 0.2, exact=True)
+    assert pairs > 100
+    gv = np.stack([g["vx"], g["vy"], g["vz"]], 1)
+    ov = np.stack([vx, vy, vz], 1)
+    assert np.abs(gv - ov).max() <= 1e-6 * np.abs(ov).max()
+
+
+# ---------------------------------------------------------------------------
+# FoF
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n_clusters,per,spread,ll", [(200, 5, 1e-3, 2e-3), (2000, 3, 5e-4, 8e-4)])
+def test_fof_labels_bit_exact(gpu, n_clusters, per, spread, ll):
+    nbx = gpu
+    d = clustered(n_clusters, per, spread, seed=11)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(p.n)
+    lab = _np(nbx.fof_labels(t, p, ll))
+    ref, groups = O.fof(d["x"], d["y"], d["z"], ll)
+    np.testing.assert_array_equal(lab, ref)
+    assert groups > 10
+
+
+def test_fof_disk_planet_formation_scale(gpu):
+    """A disk at the reference's planet_formation.json link scale (0.0261 AU)."""
+    nbx = gpu
+    d = disk(20000, seed=4)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(p.n)
+    lab = _np(nbx.fof_labels(t, p, 0.0261))
+    ref, groups = O.fof(d["x"], d["y"], d["z"], 0.0261)
+    np.testing.assert_array_equal(lab, ref)
+
+
+def test_aggregate_kats(gpu):
+    """test_aggregate.cpp:20-119 through the device partition."""
+    nbx = gpu
+    f = np.float32
+
+    def parts(pos, vel=None):
+        n = len(pos)
+        d = {k: np.zeros(n, f) for k in nbx.FIELDS}
+        for i, (a, b, c) in enumerate(pos):
+            d["x"][i], d["y"][i], d["z"][i] = a, b, c
+        if vel:
+            for i, (a, b, c) in enumerate(vel):
+                d["vx"][i], d["vy"][i], d["vz"][i] = a, b, c
+        d["mass"][:] = 3.003e-7
+        return nbx.to_gpu(d)
+
+    def agg(p, ll):
+        return nbx.detectAggregates(nbx.allocateOctree(p.n), p, ll, 3.003e-7)
+
+    a = agg(parts([(0, 0, 0), (1e-4, 0, 0), (0, 1e-4, 0), (1e-4, 1e-4, 0), (0, 0, 1e-4),
+                   (3, 0, 0), (-4, 0, 0)]), 0.002)
+    assert len(a) == 1 and a[0].n_particles == 5
+    assert agg(parts([(0, 0, 0), (10, 0, 0), (20, 0, 0)]), 0.01) == []
+    a = agg(parts([(0, 0, 0), (1e-3, 0, 0), (0, 1e-3, 0), (10, 0, 0), (10.001, 0, 0),
+                   (10, 1e-3, 0)]), 0.01)
+    assert len(a) == 2 and all(x.n_particles == 3 for x in a)
+    a = agg(parts([(1, 0, 0), (3, 0, 0)], [(0.5, 0, 0), (1.5, 0, 0)]), 5.0)
+    assert len(a) == 1
+    x = a[0]
+    assert x.n_particles == 2
+    assert abs(x.cx - 2.0) <= 1e-5 and abs(x.cy) <= 1e-5 and abs(x.cz) <= 1e-5
+    assert abs(x.mass_msun - 2 * 3.003e-7) <= 1e-4 * 2 * 3.003e-7
+    assert abs(x.dist_center - 2.0) <= 1e-5
+    assert abs(x.vx - 1.0) <= 1e-5 and abs(x.vy) <= 1e-5 and abs(x.vz) <= 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Whole step and batching
+# ---------------------------------------------------------------------------
+def _c1_params(nbx, perturber=False):
+    m = np.float32(1e-5 / 4096)
+    r_coll = nbx.particlePhysicalRadius(m)
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True)
+    pert = nbx.makePerturber(perturber, 10.0, 5.2, 0.0, 1.0)
+    return sp, nbx.disk_params(r_coll, pert), r_coll
+
+
+def test_step_vs_reference_loop(gpu):
+    """20 KDK steps of main.cu:85-140 at N=4096 with drag and the perturber:
+    positions and velocities within 4e-6 (about 30 ulp at |x| ~ 4 AU,
+    |v| ~ 1.4) of the oracle's restated reference loop.  The reference sums
+    tree terms into the star-term accumulator, this build adds the tree sum
+    to the star term once, so accelerations differ in the last bits."""
+    nbx = gpu
+    d = disk(4096, mass=1e-5, seed=9)
+    sp, dp, r_coll = _c1_params(nbx, perturber=True)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(p.n)
+    nbx.init_forces(t, p, sp, dp)
+    nbx.step(t, p, sp, dp, 0, 20)
+    g = p.numpy()
+    params = O.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, r_coll=r_coll,
+                           restitution=0.0, gas_drag_rate=1e-3, perturber_enabled=1,
+                           pert_mass_mjup=10.0, pert_radius_au=5.2, collisions=1,
+                           exact_collide=1)
+    s = O.ref_steps(d, d["mass"], params, 0, 20, init_forces=True)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        err = np.abs(g[k] - s[k]).max()
+        assert err <= 4e-6, (k, err)
+
+
+def test_batched_members_equal_plain_runs(gpu):
+    """Three disks of uneven N in one batched tree: every member bit-identical
+    to its own plain run (same kernels, same per-disk order of operations)."""
+    import torch
+    nbx = gpu
+    counts = [3000, 1024, 5000]
+    masses = [0.0, 1.0, 10.0]
+    disks = [disk(n, mass=1e-5, seed=20 + i) for i, n in enumerate(counts)]
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True)
+    dps = []
+    for n, mj in zip(counts, masses):
+        r_coll = nbx.particlePhysicalRadius(np.float32(1e-5 / n))
+        dps.append(nbx.disk_params(r_coll, nbx.makePerturber(mj > 0, mj, 5.2)))
+    plain = []
+    for dd, dp in zip(disks, dps):
+        p = nbx.to_gpu(dd)
+        t = nbx.allocateOctree(p.n)
+        nbx.init_forces(t, p, sp, dp)
+        nbx.step(t, p, sp, dp, 0, 5)
+        plain.append(p.numpy())
+    cat = {k: np.concatenate([dd[k] for dd in disks]) for k in disks[0]}
+    pb = nbx.to_gpu(cat)
+    tb = nbx.allocateOctreeBatch(counts)
+    nbx.init_forces(tb, pb, sp, dps)
+    nbx.step(tb, pb, sp, dps, 0, 5)
+    gb = pb.numpy()
+    off = 0
+    for n, ref in zip(counts, plain):
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            np.testing.assert_array_equal(gb[k][off:off + n], ref[k], err_msg=k)
+        off += n
+    torch.cuda.synchronize()
+
+
+def test_argument_errors_raise(gpu):
+    nbx = gpu
+    d = disk(100)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(50)
+    with pytest.raises(RuntimeError, match="particle count"):
+        nbx.launchBBoxKernel(t, p)
+    t2 = nbx.allocateOctree(100)
+    with pytest.raises(RuntimeError, match="before"):
+        nbx.launchForceKernel(t2, p, 0.5, 1e-3, 1.0)
