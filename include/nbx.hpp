@@ -1,0 +1,206 @@
+// nbx.hpp -- C++ mirror of the reference's launcher surface over the C ABI.
+//
+// A translation unit of the reference's host code (src/main.cu, the tests)
+// that includes this header instead of BHTree.cuh / Integrator.cuh /
+// Perturber.h / Aggregate.h keeps its call sites unchanged: the same type
+// and function names, the same argument meaning, std::runtime_error on
+// failure (ParticleData.h:9-12), synchronous at return.  Only the device
+// memory comes from nbx_particles_alloc (HIP) instead of cudaMalloc.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "nbx.h"
+
+namespace nbx_ref {
+
+inline void check(int rc, const char *op)
+{
+    if (rc != NBX_OK) throw std::runtime_error(std::string(op) + ": " + nbx_last_error());
+}
+
+// ParticleData (ParticleData.h:19-32): device pointers when allocated by
+// allocateParticlesGPU, host pointers when allocated by allocateParticlesCPU.
+using ParticleData = nbx_particles;
+
+inline ParticleData allocateParticlesCPU(int n)  // ParticleData.h:35-44
+{
+    ParticleData p{};
+    p.n = n;
+    float **f[10] = {&p.x, &p.y, &p.z, &p.vx, &p.vy, &p.vz, &p.ax, &p.ay, &p.az, &p.mass};
+    for (auto *q : f) *q = new float[n]();
+    p.id = new int32_t[n]();
+    return p;
+}
+
+inline void freeParticlesCPU(ParticleData &p)  // ParticleData.h:46-57
+{
+    float *f[10] = {p.x, p.y, p.z, p.vx, p.vy, p.vz, p.ax, p.ay, p.az, p.mass};
+    for (auto *q : f) delete[] q;
+    delete[] p.id;
+    p = ParticleData{};
+}
+
+inline ParticleData allocateParticlesGPU(int n)  // ParticleData.h:61-75
+{
+    ParticleData p{};
+    check(nbx_particles_alloc(&p, n), "allocateParticlesGPU");
+    return p;
+}
+
+inline void freeParticlesGPU(ParticleData &p) { check(nbx_particles_free(&p), "freeParticlesGPU"); }
+
+inline void copyToGPU(const ParticleData &cpu, ParticleData &gpu)  // ParticleData.h:90-105
+{
+    check(nbx_copy_to_device(&cpu, &gpu), "copyToGPU");
+}
+
+inline void copyToCPU(const ParticleData &gpu, ParticleData &cpu)  // ParticleData.h:108-123
+{
+    check(nbx_copy_to_host(&gpu, &cpu), "copyToCPU");
+}
+
+// OctreeData (BHTree.cuh:13-36): the tree scratch lives behind the handle.
+struct OctreeData {
+    nbx_tree *h = nullptr;
+    int n_bodies = 0;
+};
+
+inline OctreeData allocateOctree(int n_bodies)  // BHTree.cu:358-381
+{
+    OctreeData t;
+    check(nbx_tree_create(&t.h, n_bodies), "allocateOctree");
+    t.n_bodies = n_bodies;
+    return t;
+}
+inline void freeOctree(OctreeData &t) { check(nbx_tree_destroy(t.h), "freeOctree"); t.h = nullptr; }
+inline void resetOctree(OctreeData &t) { check(nbx_tree_reset(t.h), "resetOctree"); }
+
+inline void launchBBoxKernel(OctreeData &t, const ParticleData &p)
+{
+    check(nbx_bbox(t.h, &p), "launchBBoxKernel");
+}
+inline void launchBuildKernel(OctreeData &t, const ParticleData &p)
+{
+    check(nbx_build(t.h, &p), "launchBuildKernel");
+}
+inline void launchSummarizeKernel(OctreeData &t) { check(nbx_summarize(t.h), "launchSummarizeKernel"); }
+inline void launchSortKernel(OctreeData &t) { check(nbx_sort(t.h), "launchSortKernel"); }
+inline void launchForceKernel(OctreeData &t, ParticleData &p, float theta, float softening,
+                              float star_mass)
+{
+    check(nbx_force(t.h, &p, theta, softening, star_mass), "launchForceKernel");
+}
+inline void launchCollisionKernel(OctreeData &t, ParticleData &p, float r_coll, float restitution)
+{
+    check(nbx_collide(t.h, &p, r_coll, restitution), "launchCollisionKernel");
+}
+
+// Integrator.cuh:9-22
+inline void launchHalfKickKernel(ParticleData &g, float dt_half)
+{
+    check(nbx_half_kick(&g, dt_half), "launchHalfKickKernel");
+}
+inline void launchDriftKernel(ParticleData &g, float dt) { check(nbx_drift(&g, dt), "launchDriftKernel"); }
+inline void launchResetAccelerationKernel(ParticleData &g)
+{
+    check(nbx_reset_acceleration(&g), "launchResetAccelerationKernel");
+}
+inline void launchGasDragKernel(ParticleData &g, float drag_rate, float dt, float star_mass_msun)
+{
+    check(nbx_gas_drag(&g, drag_rate, dt, star_mass_msun), "launchGasDragKernel");
+}
+
+// Perturber.h:27-75 (host-side orbit in float, device pull)
+constexpr float MJUP_TO_MSUN = 9.5479e-4f;
+struct Perturber {
+    bool enabled = false;
+    float r = 0.f, mass_msun = 0.f, phase0 = 0.f, omega = 0.f;
+    float x = 0.f, y = 0.f, z = 0.f;
+};
+inline Perturber makePerturber(bool enabled, float mass_mjup, float radius_au, float phase0,
+                               float star_mass_msun)
+{
+    Perturber p;
+    p.enabled = enabled;
+    if (!enabled) return p;
+    p.r = radius_au;
+    p.mass_msun = mass_mjup * MJUP_TO_MSUN;
+    p.phase0 = phase0;
+    p.omega = std::sqrt(star_mass_msun / (p.r * p.r * p.r));
+    p.x = p.r * std::cos(p.phase0);
+    p.z = p.r * std::sin(p.phase0);
+    return p;
+}
+inline void updatePerturber(Perturber &p, float sim_time)
+{
+    if (!p.enabled) return;
+    float phi = p.phase0 + p.omega * sim_time;
+    p.x = p.r * std::cos(phi);
+    p.z = p.r * std::sin(phi);
+    p.y = 0.f;
+}
+inline void launchPerturberForceKernel(ParticleData &g, float px, float py, float pz, float pmass,
+                                       float softening)
+{
+    check(nbx_perturber_force(&g, px, py, pz, pmass, softening), "launchPerturberForceKernel");
+}
+
+// Aggregate.h:4-18, on device particles: the partition comes from the GPU,
+// the per-group sums follow Aggregate.cpp:58-69 on the host.
+struct Aggregate {
+    int id = 0, n_particles = 0;
+    float mass_msun = 0.f, cx = 0.f, cy = 0.f, cz = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
+    float dist_center = 0.f;
+    std::vector<int> particle_ids;
+};
+
+// `gpu` holds device pointers, `host` a host copy of the same state (the
+// reference calls detectAggregates after copyToCPU, main.cu:144-148).
+inline std::vector<Aggregate> detectAggregates(OctreeData &t, const ParticleData &gpu,
+                                               const ParticleData &host, float link_length,
+                                               float particle_mass_msun)
+{
+    const int n = gpu.n;
+    // device label buffer: the id array of a scratch ParticleData
+    ParticleData lab{};
+    check(nbx_particles_alloc(&lab, n), "detectAggregates labels");
+    check(nbx_fof(t.h, &gpu, link_length, lab.id), "detectAggregates");
+    ParticleData lab_host = allocateParticlesCPU(n);
+    check(nbx_copy_to_host(&lab, &lab_host), "detectAggregates labels");
+    std::unordered_map<int, std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) groups[lab_host.id[i]].push_back(i);
+    freeParticlesCPU(lab_host);
+    check(nbx_particles_free(&lab), "detectAggregates labels");
+    std::vector<int> roots;
+    for (auto &g : groups)
+        if (g.second.size() >= 2) roots.push_back(g.first);
+    std::sort(roots.begin(), roots.end());
+    std::vector<Aggregate> out;
+    for (int r : roots) {
+        const std::vector<int> &m = groups[r];
+        Aggregate a;
+        a.id = (int)out.size();
+        a.n_particles = (int)m.size();
+        a.particle_ids = m;
+        a.mass_msun = a.n_particles * particle_mass_msun;
+        for (int idx : m) {
+            a.cx += host.x[idx]; a.cy += host.y[idx]; a.cz += host.z[idx];
+            a.vx += host.vx[idx]; a.vy += host.vy[idx]; a.vz += host.vz[idx];
+        }
+        float inv = 1.0f / static_cast<float>(a.n_particles);
+        a.cx *= inv; a.cy *= inv; a.cz *= inv;
+        a.vx *= inv; a.vy *= inv; a.vz *= inv;
+        a.dist_center = std::sqrt(a.cx * a.cx + a.cy * a.cy + a.cz * a.cz);
+        out.push_back(std::move(a));
+    }
+    return out;
+}
+
+}  // namespace nbx_ref

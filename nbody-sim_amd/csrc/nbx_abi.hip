@@ -106,7 +106,8 @@ int alloc_bufs(nbx_tree *t)
     Bufs &B = t->B;
     const int n = t->T.n_total;
     B.n = n;
-    B.n_tiles = (n + kSortTile - 1) / kSortTile;
+    B.n_sweep_tiles = (n + kSweepTile - 1) / kSweepTile;
+    B.n_scan_tiles = (n + kScanTile - 1) / kScanTile;
     int rc = NBX_OK;
 #define ALLOC(ptr, cnt) \
     if ((rc = dmalloc(&(ptr), (size_t)(cnt))) != NBX_OK) return rc
@@ -114,32 +115,43 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.keys[1], n);
     ALLOC(B.vals[0], n);
     ALLOC(B.vals[1], n);
-    ALLOC(B.hist, 256 * B.n_tiles);
+    ALLOC(B.ghist, kPasses * 256);
+    ALLOC(B.status[0], 256 * B.n_sweep_tiles);
+    ALLOC(B.status[1], 256 * B.n_sweep_tiles);
+    ALLOC(B.tickets, 32);
+    ALLOC(B.prefix, n + 1);
+    ALLOC(B.pflag, B.n_scan_tiles);
+    ALLOC(B.pagg, B.n_scan_tiles);
+    ALLOC(B.pinc, B.n_scan_tiles);
     ALLOC(B.bodies, n);
     ALLOC(B.left, n);
     ALLOC(B.right, n);
-    ALLOC(B.par_int, n);
-    ALLOC(B.par_leaf, n);
     ALLOC(B.plen, n);
-    ALLOC(B.flags, n);
+    ALLOC(B.oct, n);
     ALLOC(B.cnt, n);
     ALLOC(B.first, n);
-    ALLOC(B.alloc, kMaxSeg);
+    ALLOC(B.alloc, 2 * kMaxSeg);
+    B.maxlev = B.alloc + kMaxSeg;  // adjacent: cleared by one fill
     ALLOC(B.bin_com, n);
+    ALLOC(B.bin_cell, n);
     B.n_rec = t->T.noff[t->T.nseg] + kRecPad;
     ALLOC(B.rec_a, B.n_rec);
     ALLOC(B.rec_b, B.n_rec);
+    ALLOC(B.rec_c, B.n_rec);
     ALLOC(B.partial, 6 * t->T.blk[t->T.nseg]);
     ALLOC(B.roots, kMaxSeg);
     ALLOC(B.dv, 3 * n);
     ALLOC(B.err, 4);
     ALLOC(B.labels, n);
-    ALLOC(B.maxlev, kMaxSeg);
 #undef ALLOC
     HIPCHK(hipMemset(B.rec_a, 0, sizeof(float4) * B.n_rec));
     HIPCHK(hipMemset(B.rec_b, 0, sizeof(int4) * B.n_rec));
+    HIPCHK(hipMemset(B.rec_c, 0, sizeof(float4) * B.n_rec));
     HIPCHK(hipMemset(B.err, 0, 4 * sizeof(int)));
     HIPCHK(hipMemset(B.dv, 0, 3 * sizeof(float) * n));
+    HIPCHK(hipMemset(B.ghist, 0, kPasses * 256 * sizeof(int)));
+    HIPCHK(hipMemset(B.tickets, 0, 32 * sizeof(int)));
+    HIPCHK(hipMemset(B.alloc, 0, 2 * kMaxSeg * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&t->h_err, 4 * sizeof(int)));
     return NBX_OK;
 }
@@ -147,10 +159,11 @@ int alloc_bufs(nbx_tree *t)
 void free_bufs(nbx_tree *t)
 {
     Bufs &B = t->B;
-    void *ptrs[] = {B.keys[0], B.keys[1], B.vals[0], B.vals[1], B.hist,    B.bodies, B.left,
-                    B.right,   B.par_int, B.par_leaf, B.plen,  B.flags,   B.cnt,    B.first,
-                    B.alloc,   B.bin_com, B.rec_a,   B.rec_b,  B.partial, B.roots,  B.dv,
-                    B.err,     B.labels,  B.maxlev};
+    void *ptrs[] = {B.keys[0], B.keys[1], B.vals[0], B.vals[1], B.ghist,   B.status[0],
+                    B.status[1], B.tickets, B.prefix, B.pflag,  B.pagg,    B.pinc,
+                    B.bodies,  B.left,    B.right,   B.plen,    B.oct,     B.cnt,
+                    B.first,   B.alloc,   B.bin_com, B.bin_cell, B.rec_a,  B.rec_b,
+                    B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
@@ -386,7 +399,8 @@ int nbx_build(nbx_tree *t, const nbx_particles *p)
 {
     int rc = check_tree_particles(t, p);
     if (rc) return rc;
-    if (t->stage < kBBox) return fail(NBX_E_STATE, "launchBuildKernel before launchBBoxKernel");
+    // the build consumes the counters and root cube the bbox pass prepares
+    if (t->stage != kBBox) return fail(NBX_E_STATE, "launchBuildKernel must follow launchBBoxKernel");
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
     t->stage = kBuilt;
     return finish(t);

@@ -9,33 +9,48 @@ namespace nbx {
 // disks; see nbx_internal.hpp for the layout.
 struct Bufs {
     int n = 0;
-    int n_tiles = 0;             // radix-sort tiles (kSortTile keys each)
+    int n_sweep_tiles = 0;       // radix-sort tiles (kSweepTile keys each)
+    int n_scan_tiles = 0;        // prefix-scan tiles (kScanTile bodies each)
     uint64_t *keys[2] = {};      // double-buffered sort keys
     int *vals[2] = {};           // double-buffered body indices
-    int *hist = nullptr;         // [256 * n_tiles] digit-major tile counts
+    int *ghist = nullptr;        // [kPasses * 256] global digit histograms
+    unsigned *status[2] = {};    // [n_sweep_tiles * 256] look-back words per pass
+    int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
+    double4 *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
+    int *pflag = nullptr;        // [n_scan_tiles] scan look-back flags
+    double4 *pagg = nullptr;     // [n_scan_tiles] tile aggregates
+    double4 *pinc = nullptr;     // [n_scan_tiles] inclusive tile prefixes
     float4 *bodies = nullptr;    // [n] sorted {x,y,z,m}
     int *left = nullptr, *right = nullptr;   // [n] binary children
-    int *par_int = nullptr;      // [n] parent of internal node
-    int *par_leaf = nullptr;     // [n] parent of sorted leaf
     int *plen = nullptr;         // [n] common-prefix length of internal node
-    int *flags = nullptr;        // [n] COM climb arrival counters
+    int *oct = nullptr;          // [n] 1 if the binary node is an octree node
     int *cnt = nullptr;          // [n] octree child count (0: not an octree node)
     int *first = nullptr;        // [n] first child record of octree node
     int *alloc = nullptr;        // [kMaxSeg] record allocators per disk
+    int *maxlev = nullptr;       // [kMaxSeg] deepest octree level per disk
     float4 *bin_com = nullptr;   // [n] {com, mass} of binary nodes
+    float4 *bin_cell = nullptr;  // [n] {cell centre, half size} of binary nodes
     float4 *rec_a = nullptr;     // walk records {com, mass}
     int4 *rec_b = nullptr;       // walk records {h, h^2, first, count}
+    float4 *rec_c = nullptr;     // walk records {cell centre | body, h}
     int n_rec = 0;
     float *partial = nullptr;    // [blocks * 6] bbox partials
     Root *roots = nullptr;       // [kMaxSeg]
     float *dv = nullptr;         // [3 n] collision velocity deltas
     int *err = nullptr;          // [4] error flags / counters
     int *labels = nullptr;       // [n] FoF scratch
-    int *maxlev = nullptr;       // [kMaxSeg] deepest octree level per disk
 };
 
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kBlock * kSortItems;
+constexpr int kPasses = 8;       // 8-bit digits over the 63-bit key
+constexpr int kSweepItems = 4;
+constexpr int kSweepTile = kBlock * kSweepItems;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+// tickets[] slots
+constexpr int kTicketPass = 0;   // + pass (9 slots incl. the disk pass)
+constexpr int kTicketScan = 10;
+constexpr int kBlockCtr = 11;    // last-block counter of the bbox reduction
+
 
 // ---- integrator (reference-identical float ops) --------------------------
 void launch_half_kick(float *vx, float *vy, float *vz, const float *ax, const float *ay,

@@ -1,10 +1,11 @@
 // nbx_tree.hip -- octree construction and integrator kernels for gfx950.
 //
 // Build pipeline (one launch per phase for all disks of a batch):
-//   bbox partials -> root cube -> 63-bit keys -> LSD radix sort (8 x 8-bit)
-//   -> gather sorted bodies -> binary radix tree (one thread per internal
-//   node, longest-common-prefix search) -> COM bottom-up (second arrival
-//   climbs) -> compressed octree records (children contiguous).
+//   bbox partials + root cube (last block) -> 63-bit keys + digit histograms
+//   -> 8 stable radix passes with decoupled look-back (the last one gathers
+//   the sorted bodies) -> FP64 prefix scan of {m, m x, m y, m z} -> binary
+//   radix tree (one thread per internal node) with node mass / COM from the
+//   prefix, octree flags and cell geometry -> record allocation -> records.
 // Every phase is HBM-streaming integer/float work; nothing here is a GEMM.
 #include <float.h>
 
@@ -81,10 +82,13 @@ __global__ void k_perturber(float *ax, float *ay, float *az, const float *__rest
 
 // Fused first half-kick + drift (main.cu:104-107) with the bbox partial of
 // the drifted positions (BHTree.cu:94-133) in the same pass over HBM.
+__device__ void bbox_finish(const SegTable &T, float lx, float ly, float lz, float hx, float hy,
+                            float hz, float *partial, Root *roots, int *tickets, int *ghist);
+
 __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox(
     SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
     const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
-    float dt_half, float dt, float *partial)
+    float dt_half, float dt, float *partial, Root *roots, int *tickets, int *ghist)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
@@ -105,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox(
         ly = hy = yi;
         lz = hz = zi;
     }
-    block_minmax6(lx, ly, lz, hx, hy, hz, partial + 6 * b);
+    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, ghist);
 }
 
 // Collision delta, second half-kick and gas drag (main.cu:131-139 after the
@@ -148,12 +152,69 @@ __global__ void k_fill_float(float *p, float v, int n)
 }
 
 // ===========================================================================
-// Bounding box and root cube
+// Bounding box and root cube.  Every block writes its min/max partial; the
+// last block to finish reduces the partials of each disk, pads the box by
+// 1.001 about its midpoint (BHTree.cu:440-446), forms the root cube
+// (BHTree.cu:187-191) and clears the counters of the build that follows.
 // ===========================================================================
+
+__device__ void root_from_box(const float *lo_hi, Root *out)
+{
+#pragma clang fp contract(off)
+    float bmin[3], bmax[3];
+    for (int k = 0; k < 3; ++k) {
+        float lo = lo_hi[k], hi = lo_hi[3 + k];
+        float mid = (lo + hi) * 0.5f;
+        float half = (hi - lo) * 0.5f * 1.001f;
+        bmin[k] = mid - half;
+        bmax[k] = mid + half;
+    }
+    Root R;
+    R.c[0] = (bmin[0] + bmax[0]) * 0.5f;
+    R.c[1] = (bmin[1] + bmax[1]) * 0.5f;
+    R.c[2] = (bmin[2] + bmax[2]) * 0.5f;
+    R.c[3] = fmaxf(fmaxf(bmax[0] - bmin[0], bmax[1] - bmin[1]), bmax[2] - bmin[2]) * 0.5f * 1.001f;
+    *out = R;
+}
+
+__device__ void bbox_finish(const SegTable &T, float lx, float ly, float lz, float hx, float hy,
+                            float hz, float *partial, Root *roots, int *tickets, int *)
+{
+    __shared__ int s_last;
+    __shared__ float s_box[6];
+    block_minmax6(lx, ly, lz, hx, hy, hz, partial + 6 * blockIdx.x);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int prev = atomicAdd(&tickets[kBlockCtr], 1);
+        s_last = prev == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    for (int s = 0; s < T.nseg; ++s) {
+        float a = FLT_MAX, b = FLT_MAX, c = FLT_MAX, d = -FLT_MAX, e = -FLT_MAX, f = -FLT_MAX;
+        for (int q = T.blk[s] + threadIdx.x; q < T.blk[s + 1]; q += kBlock) {
+            const float *p = partial + 6 * q;
+            a = fminf(a, __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            b = fminf(b, __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            c = fminf(c, __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            d = fmaxf(d, __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            e = fmaxf(e, __hip_atomic_load(p + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            f = fmaxf(f, __hip_atomic_load(p + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+        block_minmax6(a, b, c, d, e, f, s_box);
+        __syncthreads();
+        if (threadIdx.x == 0) root_from_box(s_box, &roots[s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tickets[kBlockCtr] = 0;
+}
 
 __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__restrict__ x,
                                                  const float *__restrict__ y,
-                                                 const float *__restrict__ z, float *partial)
+                                                 const float *__restrict__ z, float *partial,
+                                                 Root *roots, int *tickets, int *ghist)
 {
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
@@ -165,176 +226,136 @@ __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__rest
         ly = hy = y[i];
         lz = hz = z[i];
     }
-    block_minmax6(lx, ly, lz, hx, hy, hz, partial + 6 * b);
-}
-
-// One block per disk: reduce the partials, pad by 1.001 about the midpoint
-// (BHTree.cu:440-446) and form the root cube (BHTree.cu:187-191).
-__global__ __launch_bounds__(kBlock) void k_root(SegTable T, const float *__restrict__ partial,
-                                                 Root *roots)
-{
-#pragma clang fp contract(off)
-    const int s = blockIdx.x;
-    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
-    float hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
-    for (int b = T.blk[s] + threadIdx.x; b < T.blk[s + 1]; b += kBlock) {
-        const float *p = partial + 6 * b;
-        lx = fminf(lx, p[0]); ly = fminf(ly, p[1]); lz = fminf(lz, p[2]);
-        hx = fmaxf(hx, p[3]); hy = fmaxf(hy, p[4]); hz = fmaxf(hz, p[5]);
-    }
-    __shared__ float r[6];
-    block_minmax6(lx, ly, lz, hx, hy, hz, r);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float bmin[3], bmax[3];
-        for (int k = 0; k < 3; ++k) {
-            float lo = r[k], hi = r[3 + k];
-            float mid = (lo + hi) * 0.5f;
-            float half = (hi - lo) * 0.5f * 1.001f;
-            bmin[k] = mid - half;
-            bmax[k] = mid + half;
-        }
-        Root R;
-        R.c[0] = (bmin[0] + bmax[0]) * 0.5f;
-        R.c[1] = (bmin[1] + bmax[1]) * 0.5f;
-        R.c[2] = (bmin[2] + bmax[2]) * 0.5f;
-        R.c[3] = fmaxf(fmaxf(bmax[0] - bmin[0], bmax[1] - bmin[1]), bmax[2] - bmin[2]) * 0.5f *
-                 1.001f;
-        roots[s] = R;
-    }
+    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, ghist);
 }
 
 // ===========================================================================
 // Keys: the octant digits of BHTree.cu:164-170 along the float descent of
-// BHTree.cu:253-256, 21 levels, first level most significant.
+// BHTree.cu:253-256, 21 levels, first level most significant.  The same
+// pass accumulates the digit histograms of all radix passes and clears the
+// look-back words of the first pass and of the prefix scan.
 // ===========================================================================
 __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__restrict__ x,
                                                  const float *__restrict__ y,
                                                  const float *__restrict__ z,
                                                  const Root *__restrict__ roots,
-                                                 uint64_t *keys, int *vals)
+                                                 uint64_t *keys, int *vals, int *ghist,
+                                                 unsigned *status0, int n_status, int *pflag,
+                                                 int n_pflag, int *tickets)
 {
 #pragma clang fp contract(off)
+    __shared__ int h[kPasses * 256];
+    for (int q = threadIdx.x; q < kPasses * 256; q += kBlock) h[q] = 0;
+    const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
+    for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
+    for (int q = gid; q < n_pflag; q += gsz) pflag[q] = 0;
+    if (gid <= kTicketScan) tickets[gid] = 0;  // tickets of the radix passes and the scan
+    __syncthreads();
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
     const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
-    if (i >= T.off[s + 1]) return;
-    const Root R = roots[s];
-    float cx = R.c[0], cy = R.c[1], cz = R.c[2], h = R.c[3];
-    const float px = x[i], py = y[i], pz = z[i];
-    uint64_t key = 0;
+    if (i < T.off[s + 1]) {
+        const Root R = roots[s];
+        float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
+        const float px = x[i], py = y[i], pz = z[i];
+        uint64_t key = 0;
 #pragma unroll
-    for (int l = 0; l < kLevels; ++l) {
-        const bool ox = px > cx, oy = py > cy, oz = pz > cz;
-        key = (key << 3) | (uint64_t)((ox ? 1 : 0) | (oy ? 2 : 0) | (oz ? 4 : 0));
-        h = h * 0.5f;
-        cx = cx + (ox ? h : -h);
-        cy = cy + (oy ? h : -h);
-        cz = cz + (oz ? h : -h);
+        for (int l = 0; l < kLevels; ++l) {
+            const bool ox = px > cx, oy = py > cy, oz = pz > cz;
+            key = (key << 3) | (uint64_t)((ox ? 1 : 0) | (oy ? 2 : 0) | (oz ? 4 : 0));
+            hh = hh * 0.5f;
+            cx = cx + (ox ? hh : -hh);
+            cy = cy + (oy ? hh : -hh);
+            cz = cz + (oz ? hh : -hh);
+        }
+        keys[i] = key;
+        vals[i] = i;
+#pragma unroll
+        for (int p = 0; p < kPasses; ++p) atomicAdd(&h[p * 256 + (int)((key >> (8 * p)) & 255u)], 1);
     }
-    keys[i] = key;
-    vals[i] = i;
+    __syncthreads();
+    for (int q = threadIdx.x; q < kPasses * 256; q += kBlock)
+        if (h[q]) atomicAdd(&ghist[q], h[q]);
 }
 
 // ===========================================================================
-// LSD radix sort, 8-bit digits, stable.  Per pass: tile histograms
-// (digit-major), one exclusive scan, stable scatter (wave match + LDS).
-// mode 0: digit = (key >> shift) & 255;  mode 1: digit = disk of the body.
+// One stable radix pass (8-bit digit) with decoupled look-back: tiles take
+// tickets in launch order, rank their keys locally (wave match + LDS), post
+// their per-digit counts, and add the counts of earlier tiles read back
+// through one status word per (tile, digit): bit 31 inclusive prefix,
+// bit 30 tile aggregate, low 30 bits the count.
+// mode 0: digit = (key >> 8 pass) & 255;  mode 1: digit = disk of the body.
 // ===========================================================================
+constexpr unsigned kAgg = 1u << 30, kInc = 1u << 31, kValMask = kAgg - 1;
 
-__device__ inline int sort_digit(int mode, uint64_t key, int val, int shift, const SegTable &T)
+__device__ inline int sweep_digit(int mode, uint64_t key, int val, int shift, const SegTable &T)
 {
     return mode == 0 ? (int)((key >> shift) & 255u) : seg_of_body(T, val);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sort_hist(const uint64_t *__restrict__ keys,
-                                                      const int *__restrict__ vals, int n,
-                                                      int shift, int mode, SegTable T,
-                                                      int n_tiles, int *hist)
+// Exclusive scan over the 256 threads of a block; returns the exclusive
+// prefix of v, *total receives the sum.
+__device__ inline int block_excl_scan256(int v, int *total)
 {
-    __shared__ int h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const int base = blockIdx.x * kSortTile;
+    __shared__ int s_w[kBlock / kWave];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int inc = v;
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        int i = base + r * kBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[sort_digit(mode, keys[i], vals[i], shift, T)], 1);
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) {
+        const int c = s_w[k];
+        if (k < w) before += c;
+        tot += c;
     }
     __syncthreads();
-    hist[threadIdx.x * n_tiles + blockIdx.x] = h[threadIdx.x];
+    *total = tot;
+    return before + inc - v;
 }
 
-// Single-block exclusive scan in place, chunks of 4 * 1024 with a carry.
-__global__ __launch_bounds__(1024) void k_scan1(int *data, int n)
+__global__ __launch_bounds__(kBlock) void k_sweep(
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *kout, int *vout,
+    int n, int pass, int mode, SegTable T, const int *__restrict__ ghist, unsigned *status,
+    unsigned *status_next, int *tickets, int gather, const float *__restrict__ x,
+    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ m,
+    float4 *bodies)
 {
-    __shared__ int wsum[16];
-    __shared__ int carry_s;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (t == 0) carry_s = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 4096) {
-        int v[4];
-        int sum = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int i = base + 4 * t + k;
-            v[k] = i < n ? data[i] : 0;
-            sum += v[k];
-        }
-        int inc = sum;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int y = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += y;
-        }
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        if (t < 16) {
-            int x = wsum[t];
-            int acc = x;
-            for (int o = 1; o < 16; o <<= 1) {
-                int y = __shfl_up(acc, o, 16);
-                if (t >= o) acc += y;
-            }
-            wsum[t] = acc - x;  // exclusive prefix of the wave sums
-        }
-        __syncthreads();
-        int run = carry_s + wsum[w] + inc - sum;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            int i = base + 4 * t + k;
-            if (i < n) data[i] = run;
-            run += v[k];
-        }
-        __syncthreads();
-        if (t == 1023) carry_s = run;
-        __syncthreads();
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_sort_scatter(
-    const uint64_t *__restrict__ keys_in, const int *__restrict__ vals_in, uint64_t *keys_out,
-    int *vals_out, int n, int shift, int mode, SegTable T, int n_tiles,
-    const int *__restrict__ hist)
-{
-    __shared__ int s_off[256];
+    __shared__ int s_goff[256], s_cnt[256], s_pre[256];
     __shared__ int s_wc[kBlock / kWave][256];
-    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
-    s_off[t] = hist[t * n_tiles + blockIdx.x];
-    const int base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
+    __shared__ int s_tile;
+    const int t = threadIdx.x, w = t >> 6;
+    if (t == 0) s_tile = atomicAdd(&tickets[kTicketPass + pass], 1);
+    const int g = mode == 0 ? ghist[pass * 256 + t]
+                            : (t < T.nseg ? T.off[t + 1] - T.off[t] : 0);
+    int total;
+    s_goff[t] = block_excl_scan256(g, &total);
+    s_cnt[t] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    if (status_next) status_next[tile * 256 + t] = 0;
+    const int shift = 8 * pass;
+    const int base = tile * kSweepTile;
+    uint64_t kk[kSweepItems];
+    int vv[kSweepItems], dd[kSweepItems], lo[kSweepItems];
 #pragma unroll
-        for (int k = 0; k < kBlock / kWave; ++k) s_wc[k][t] = 0;
+    for (int r = 0; r < kSweepItems; ++r) {
+#pragma unroll
+        for (int q = 0; q < kBlock / kWave; ++q) s_wc[q][t] = 0;
         __syncthreads();
         const int i = base + r * kBlock + t;
         const bool valid = i < n;
         uint64_t key = 0;
         int val = 0, d = 0;
         if (valid) {
-            key = keys_in[i];
-            val = vals_in[i];
-            d = sort_digit(mode, key, val, shift, T);
+            key = kin[i];
+            val = vin[i];
+            d = sweep_digit(mode, key, val, shift, T);
         }
         uint64_t peers = ballot64(valid);
 #pragma unroll
@@ -346,9 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
         const int rank = __popcll(peers & lanemask_lt());
         if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
         __syncthreads();
-        int total;
         {
-            int c0 = s_wc[0][t], c1 = s_wc[1][t], c2 = s_wc[2][t], c3 = s_wc[3][t];
+            const int c0 = s_wc[0][t], c1 = s_wc[1][t], c2 = s_wc[2][t], c3 = s_wc[3][t];
             s_wc[0][t] = 0;
             s_wc[1][t] = c0;
             s_wc[2][t] = c0 + c1;
@@ -356,50 +376,179 @@ __global__ __launch_bounds__(kBlock) void k_sort_scatter(
             total = c0 + c1 + c2 + c3;
         }
         __syncthreads();
-        if (valid) {
-            const int pos = s_off[d] + s_wc[w][d] + rank;
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+        kk[r] = key;
+        vv[r] = val;
+        dd[r] = valid ? d : -1;
+        lo[r] = valid ? s_cnt[d] + s_wc[w][d] + rank : 0;
+        __syncthreads();
+        s_cnt[t] += total;
+    }
+    __syncthreads();
+    // publish this tile's count of digit t and look back over earlier tiles
+    const unsigned cnt = (unsigned)s_cnt[t];
+    unsigned *mine = status + tile * 256 + t;
+    unsigned pre = 0;
+    if (tile == 0) {
+        __hip_atomic_store(mine, kInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(mine, kAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int j = tile - 1;
+        while (true) {
+            const unsigned v = __hip_atomic_load(status + j * 256 + t, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (v == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            pre += v & kValMask;
+            if (v & kInc) break;
+            --j;
         }
-        __syncthreads();
-        s_off[t] += total;
-        __syncthreads();
+        __hip_atomic_store(mine, kInc | (pre + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_pre[t] = (int)pre;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSweepItems; ++r) {
+        const int d = dd[r];
+        if (d < 0) continue;
+        const int pos = s_goff[d] + s_pre[d] + lo[r];
+        kout[pos] = kk[r];
+        vout[pos] = vv[r];
+        if (gather) {
+            const int v = vv[r];
+            bodies[pos] = make_float4(x[v], y[v], z[v], m[v]);
+        }
     }
 }
 
-// Sorted bodies {x,y,z,m} for the walk leaves.
-__global__ __launch_bounds__(kBlock) void k_gather(const int *__restrict__ perm,
-                                                   const float *__restrict__ x,
-                                                   const float *__restrict__ y,
-                                                   const float *__restrict__ z,
-                                                   const float *__restrict__ m, float4 *bodies,
-                                                   int n)
+// ===========================================================================
+// Exclusive prefix P[i] = sum_{s<i} {m, m x, m y, m z} of the sorted bodies
+// in double precision (single pass, decoupled look-back).  Node mass and
+// COM follow as differences over the node's range of sorted positions.
+// ===========================================================================
+__device__ inline double4 d4add(double4 a, double4 b)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int i = perm[s];
-    bodies[s] = make_float4(x[i], y[i], z[i], m[i]);
+    return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__device__ inline double4 d4shfl_up(double4 v, int o)
+{
+    return make_double4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64),
+                        __shfl_up(v.w, o, 64));
+}
+
+__global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bodies, int n,
+                                                   double4 *P, int *pflag, double4 *pagg,
+                                                   double4 *pinc, int *tickets, int *ghist)
+{
+    __shared__ int s_tile;
+    __shared__ double4 s_w[kBlock / kWave];
+    __shared__ double4 s_pre;
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+    if (t == 0) s_tile = atomicAdd(&tickets[kTicketScan], 1);
+    __syncthreads();
+    const int tile = s_tile;
+    const int base = tile * kScanTile + t * kScanItems;
+    double4 v[kScanItems];
+    double4 sum = make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int i = base + k;
+        double4 e = make_double4(0, 0, 0, 0);
+        if (i < n) {
+            const float4 b = bodies[i];
+            const double mm = b.w;
+            e = make_double4(mm, mm * b.x, mm * b.y, mm * b.z);
+        }
+        v[k] = e;
+        sum = d4add(sum, e);
+    }
+    double4 inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double4 y = d4shfl_up(inc, o);
+        if (lane >= o) inc = d4add(inc, y);
+    }
+    double4 excl = d4shfl_up(inc, 1);
+    if (lane == 0) excl = make_double4(0, 0, 0, 0);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    double4 before = make_double4(0, 0, 0, 0), agg = make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < kBlock / kWave; ++k) {
+        if (k < w) before = d4add(before, s_w[k]);
+        agg = d4add(agg, s_w[k]);
+    }
+    if (t == 0) {
+        double4 pre = make_double4(0, 0, 0, 0);
+        if (tile == 0) {
+            pinc[0] = agg;
+            __hip_atomic_store(&pflag[0], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            pagg[tile] = agg;
+            __hip_atomic_store(&pflag[tile], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1;
+            while (true) {
+                const int f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+                if (f == 0) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if (f == 2) {
+                    pre = d4add(pre, pinc[j]);
+                    break;
+                }
+                pre = d4add(pre, pagg[j]);
+                --j;
+            }
+            pinc[tile] = d4add(pre, agg);
+            __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_pre = pre;
+        if (tile == 0) P[0] = make_double4(0, 0, 0, 0);
+    }
+    if (tile == 0)  // histograms are consumed: clear them for the next build
+        for (int q = t; q < kPasses * 256; q += kBlock) ghist[q] = 0;
+    __syncthreads();
+    double4 run = d4add(s_pre, d4add(before, excl));
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        run = d4add(run, v[k]);
+        if (base + k < n) P[base + k + 1] = run;
+    }
 }
 
 // ===========================================================================
-// Binary radix tree over the sorted keys of each disk (longest common prefix
-// search, ties broken by position).  Internal node j of disk s covers a
-// range of sorted positions; children are encoded >= 0 (internal, global
-// index) or < 0 (leaf, ~global sorted position).
+// Binary radix tree over the sorted keys of each disk (longest common
+// prefix search, ties broken by position).  Internal node j of disk s covers
+// sorted positions [lo, hi]; children are encoded >= 0 (internal, global
+// index) or < 0 (leaf, ~global sorted position).  The same thread derives
+// the node's mass and COM from the prefix, its octree level
+// floor(prefix bits / 3), whether it starts a new octree cell (its parent's
+// prefix is max(lcp(lo-1, lo), lcp(hi, hi+1))), and the cell's centre by
+// the float descent of the keys.
 // ===========================================================================
 
 __device__ inline int lcp(const uint64_t *__restrict__ k, int n, int a, int b)
 {
-    if (b < 0 || b >= n) return -1;
-    uint64_t ka = k[a], kb = k[b];
+    if (a < 0 || b < 0 || a >= n || b >= n) return -1;
+    const uint64_t ka = k[a], kb = k[b];
     if (ka == kb) return 64 + __clz((unsigned)(a ^ b));
     return __clzll(ka ^ kb);
 }
 
-__global__ __launch_bounds__(kBlock) void k_karras(SegTable T, const uint64_t *__restrict__ keys,
-                                                   int *left, int *right, int *par_int,
-                                                   int *par_leaf, int *plen)
+// Octree level of a binary node: floor(prefix bits / 3); identical keys
+// (prefix >= 64) form level-21 cells.  Key bit 63 is always zero.
+__device__ inline int oct_level(int plen) { return plen >= 64 ? kLevels : (plen - 1) / 3; }
+
+__global__ __launch_bounds__(kBlock) void k_karras(
+    SegTable T, const uint64_t *__restrict__ keys, const double4 *__restrict__ P,
+    const float4 *__restrict__ bodies, const Root *__restrict__ roots, int *left, int *right,
+    int *plen, int *octf, float4 *bin_com, float4 *bin_cell)
 {
+#pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= T.ioff[T.nseg]) return;
     const int s = seg_of_internal(T, j);
@@ -422,89 +571,44 @@ __global__ __launch_bounds__(kBlock) void k_karras(SegTable T, const uint64_t *_
     } while (t > 1);
     const int gamma = i + sp * d + min(d, 0);
     const int lo = min(i, jj), hi = max(i, jj);
-    int lc, rc;
-    if (lo == gamma) {
-        lc = ~(T.off[s] + gamma);
-        par_leaf[T.off[s] + gamma] = j;
-    } else {
-        lc = T.ioff[s] + gamma;
-        par_int[lc] = j;
-    }
-    if (hi == gamma + 1) {
-        rc = ~(T.off[s] + gamma + 1);
-        par_leaf[T.off[s] + gamma + 1] = j;
-    } else {
-        rc = T.ioff[s] + gamma + 1;
-        par_int[rc] = j;
-    }
-    left[j] = lc;
-    right[j] = rc;
+    left[j] = lo == gamma ? ~(T.off[s] + gamma) : T.ioff[s] + gamma;
+    right[j] = hi == gamma + 1 ? ~(T.off[s] + gamma + 1) : T.ioff[s] + gamma + 1;
     plen[j] = dnode;
-    if (i == 0) par_int[j] = -1;
-}
-
-// Bottom-up mass / COM: each leaf climbs; the second thread to arrive at a
-// node combines its two children and continues.
-__global__ __launch_bounds__(kBlock) void k_com(SegTable T, const float4 *__restrict__ bodies,
-                                                const int *__restrict__ par_leaf,
-                                                const int *__restrict__ par_int,
-                                                const int *__restrict__ left,
-                                                const int *__restrict__ right, int *flags,
-                                                float4 *bin_com)
-{
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= T.n_total) return;
-    const int s = seg_of_body(T, p);
-    if (T.off[s + 1] - T.off[s] < 2) return;
-    int node = par_leaf[p];
-    while (node >= 0) {
-        int old = __hip_atomic_fetch_add(&flags[node], 1, __ATOMIC_ACQ_REL,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0) return;
-        const int lc = left[node], rc = right[node];
-        float4 a, b;
-        if (lc < 0) a = bodies[~lc];
-        else a = bin_com[lc];
-        if (rc < 0) b = bodies[~rc];
-        else b = bin_com[rc];
-        const float m = a.w + b.w;
-        float4 c;
-        if (m > 0.f) {
-            const float inv = 1.0f / m;
-            c = make_float4((a.w * a.x + b.w * b.x) * inv, (a.w * a.y + b.w * b.y) * inv,
-                            (a.w * a.z + b.w * b.z) * inv, m);
-        } else {
-            c = make_float4(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.f);
-        }
-        bin_com[node] = c;
-        node = par_int[node];
+    const int lev = oct_level(dnode);
+    const int dpar = max(lcp(k, n, lo - 1, lo), lcp(k, n, hi, hi + 1));
+    octf[j] = (dpar < 0 || oct_level(dpar) != lev || lev == kLevels) ? 1 : 0;
+    // mass and COM over sorted positions [lo, hi]
+    const double4 a = P[T.off[s] + lo], b = P[T.off[s] + hi + 1];
+    const double mm = b.x - a.x;
+    float4 c;
+    if (mm > 0.0) {
+        c = make_float4((float)((b.y - a.y) / mm), (float)((b.z - a.z) / mm),
+                        (float)((b.w - a.w) / mm), (float)mm);
+    } else {
+        const float4 p = bodies[T.off[s] + lo], q = bodies[T.off[s] + hi];
+        c = make_float4(0.5f * (p.x + q.x), 0.5f * (p.y + q.y), 0.5f * (p.z + q.z), 0.f);
     }
-}
-
-// Octree level of binary node j: floor(prefix bits / 3); identical keys
-// (prefix >= 64) form level-21 cells.
-__device__ inline int oct_level(int plen)
-{
-    return plen >= 64 ? kLevels : (plen - 1) / 3;
-}
-
-// A binary node is folded into its parent when both lie in the same digit.
-__device__ inline bool is_oct(const int *__restrict__ par_int, const int *__restrict__ plen,
-                              int j)
-{
-    const int p = par_int[j];
-    if (p < 0) return true;
-    const int lj = oct_level(plen[j]);
-    return !(lj == oct_level(plen[p]) && lj < kLevels);
+    bin_com[j] = c;
+    // cell centre: descend `lev` octant digits of this node's prefix
+    const Root R = roots[s];
+    float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
+    const uint64_t key = k[lo];
+    for (int q = 0; q < lev; ++q) {
+        const int dg = (int)((key >> (3 * (kLevels - 1 - q))) & 7u);
+        hh = hh * 0.5f;
+        cx = cx + ((dg & 1) ? hh : -hh);
+        cy = cy + ((dg & 2) ? hh : -hh);
+        cz = cz + ((dg & 4) ? hh : -hh);
+    }
+    bin_cell[j] = make_float4(cx, cy, cz, hh);
 }
 
 // Enumerate the octree children of octree node j in Morton order: DFS over
-// the folded binary descendants (at most 7 for a digit, bucket nodes are
-// binary).  f(child_code) is called for each child in order.
+// the binary descendants folded into it (at most 7 inside one digit; bucket
+// nodes of identical keys are never folded).  f(k, code) per child.
 template <class F>
 __device__ inline int oct_children(const int *__restrict__ left, const int *__restrict__ right,
-                                   const int *__restrict__ par_int,
-                                   const int *__restrict__ plen, int j, F f)
+                                   const int *__restrict__ octf, int j, F f)
 {
     int stack[16];
     int sp = 0, n = 0;
@@ -512,7 +616,7 @@ __device__ inline int oct_children(const int *__restrict__ left, const int *__re
     stack[sp++] = left[j];
     while (sp > 0) {
         const int c = stack[--sp];
-        if (c < 0 || is_oct(par_int, plen, c)) {
+        if (c < 0 || octf[c]) {
             f(n, c);
             ++n;
         } else {
@@ -524,88 +628,100 @@ __device__ inline int oct_children(const int *__restrict__ left, const int *__re
     return n;
 }
 
+// Child counts and record blocks: one ticketed atomic per block and disk.
 __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__restrict__ left,
                                                       const int *__restrict__ right,
-                                                      const int *__restrict__ par_int,
+                                                      const int *__restrict__ octf,
                                                       const int *__restrict__ plen, int *cnt,
                                                       int *first, int *alloc, int *maxlev,
                                                       int *err)
 {
+    __shared__ int s_base[kMaxSeg];
+    __shared__ int s_tot[kMaxSeg];
+    __shared__ int s_lev[kMaxSeg];
+    const int ni = T.ioff[T.nseg];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= T.ioff[T.nseg]) return;
-    if (!is_oct(par_int, plen, j)) {
-        cnt[j] = 0;
-        return;
+    if (threadIdx.x < kMaxSeg) {
+        s_tot[threadIdx.x] = 0;
+        s_lev[threadIdx.x] = 0;
     }
-    const int s = seg_of_internal(T, j);
-    const int c = oct_children(left, right, par_int, plen, j, [](int, int) {});
-    if (c < 0) {
-        atomicOr(err, kErrTree);
-        cnt[j] = 0;
-        return;
+    __syncthreads();
+    int c = 0, s = 0, off = 0;
+    if (j < ni) {
+        s = seg_of_internal(T, j);
+        if (octf[j]) {
+            c = oct_children(left, right, octf, j, [](int, int) {});
+            if (c < 0) {
+                atomicOr(err, kErrTree);
+                c = 0;
+            }
+        }
+        if (c) {
+            off = atomicAdd(&s_tot[s], c);  // LDS: order inside the block is irrelevant
+            atomicMax(&s_lev[s], oct_level(plen[j]));
+        }
     }
-    cnt[j] = c;
-    first[j] = T.noff[s] + 1 + atomicAdd(&alloc[s], c);
-    atomicMax(&maxlev[s], oct_level(plen[j]));
+    __syncthreads();
+    if (threadIdx.x < kMaxSeg && s_tot[threadIdx.x]) {
+        s_base[threadIdx.x] = atomicAdd(&alloc[threadIdx.x], s_tot[threadIdx.x]);
+        atomicMax(&maxlev[threadIdx.x], s_lev[threadIdx.x]);
+    }
+    __syncthreads();
+    if (j < ni) {
+        cnt[j] = c;
+        if (c) first[j] = T.noff[s] + 1 + s_base[s] + off;
+    }
 }
 
-// Walk records: children of octree node j into its block, and the root
-// record of each disk at noff[s].
-__global__ __launch_bounds__(kBlock) void k_oct_fill(SegTable T, const Root *__restrict__ roots,
-                                                     const int *__restrict__ left,
-                                                     const int *__restrict__ right,
-                                                     const int *__restrict__ par_int,
-                                                     const int *__restrict__ plen,
-                                                     const int *__restrict__ cnt,
-                                                     const int *__restrict__ first,
-                                                     const float4 *__restrict__ bin_com,
-                                                     const float4 *__restrict__ bodies,
-                                                     float4 *rec_a, int4 *rec_b)
+// Walk records: the children of octree node j into its block, the root
+// record of each disk at noff[s], and one-body disks (root = the body).
+__global__ __launch_bounds__(kBlock) void k_oct_fill(
+    SegTable T, const int *__restrict__ left, const int *__restrict__ right,
+    const int *__restrict__ octf, const int *__restrict__ cnt, const int *__restrict__ first,
+    const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
+    const float4 *__restrict__ bodies, float4 *rec_a, int4 *rec_b, float4 *rec_c)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= T.ioff[T.nseg]) return;
-    const int c_j = cnt[j];
-    if (c_j == 0) return;
-    const int s = seg_of_internal(T, j);
-    const float rhs = roots[s].c[3];
-    auto rec_of = [&](int code, float4 &a, int4 &b) {
+    auto rec_of = [&](int code, float4 &a, int4 &b, float4 &c) {
         if (code < 0) {
             const int p = ~code;
             a = bodies[p];
             b = make_int4(__float_as_int(0.f), __float_as_int(-1.f), p, 0);
+            c = make_float4(a.x, a.y, a.z, 0.f);
         } else {
-            const float h = ldexpf(rhs, -oct_level(plen[code]));
+            c = bin_cell[code];
             a = bin_com[code];
-            b = make_int4(__float_as_int(h), __float_as_int(h * h), first[code], cnt[code]);
+            b = make_int4(__float_as_int(c.w), __float_as_int(c.w * c.w), first[code], cnt[code]);
         }
     };
-    if (par_int[j] < 0) {
-        float4 a;
+    if (j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
+        float4 a, c;
         int4 b;
-        rec_of(j, a, b);
+        rec_of(~T.off[j], a, b, c);
+        rec_a[T.noff[j]] = a;
+        rec_b[T.noff[j]] = b;
+        rec_c[T.noff[j]] = c;
+    }
+    if (j >= T.ioff[T.nseg]) return;
+    if (cnt[j] == 0) return;
+    const int s = seg_of_internal(T, j);
+    if (j == T.ioff[s]) {  // local node 0 is the disk's root
+        float4 a, c;
+        int4 b;
+        rec_of(j, a, b, c);
         rec_a[T.noff[s]] = a;
         rec_b[T.noff[s]] = b;
+        rec_c[T.noff[s]] = c;
     }
     const int f0 = first[j];
-    oct_children(left, right, par_int, plen, j, [&](int k, int code) {
-        float4 a;
+    oct_children(left, right, octf, j, [&](int k, int code) {
+        float4 a, c;
         int4 b;
-        rec_of(code, a, b);
+        rec_of(code, a, b, c);
         rec_a[f0 + k] = a;
         rec_b[f0 + k] = b;
+        rec_c[f0 + k] = c;
     });
-}
-
-// Disks of one body: the root record is the body itself.
-__global__ void k_single_roots(SegTable T, const float4 *__restrict__ bodies, float4 *rec_a,
-                               int4 *rec_b)
-{
-    const int s = threadIdx.x;
-    if (s >= T.nseg) return;
-    if (T.off[s + 1] - T.off[s] != 1) return;
-    const int p = T.off[s];
-    rec_a[T.noff[s]] = bodies[p];
-    rec_b[T.noff[s]] = make_int4(__float_as_int(0.f), __float_as_int(-1.f), p, 0);
 }
 
 // ===========================================================================
@@ -654,7 +770,8 @@ void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, flo
                             const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
 {
     hipLaunchKernelGGL(k_kick_drift_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial);
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial, B.roots, B.tickets,
+                       B.ghist);
 }
 
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -669,65 +786,54 @@ void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const fl
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.partial);
+    hipLaunchKernelGGL(k_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.partial,
+                       B.roots, B.tickets, B.ghist);
 }
 
-void launch_root(const SegTable &T, Bufs &B, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_root, dim3(T.nseg), dim3(kBlock), 0, st, T, B.partial, B.roots);
-}
-
-static void sort_pass(const SegTable &T, Bufs &B, int &cur, int shift, int mode, hipStream_t st)
-{
-    const int n = B.n, nt = B.n_tiles;
-    hipLaunchKernelGGL(k_sort_hist, dim3(nt), dim3(kBlock), 0, st, B.keys[cur], B.vals[cur], n,
-                       shift, mode, T, nt, B.hist);
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, B.hist, 256 * nt);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(nt), dim3(kBlock), 0, st, B.keys[cur], B.vals[cur],
-                       B.keys[cur ^ 1], B.vals[cur ^ 1], n, shift, mode, T, nt, B.hist);
-    cur ^= 1;
-}
+void launch_root(const SegTable &, Bufs &, hipStream_t) {}
 
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
                   const float *mass, Bufs &B, hipStream_t st)
 {
     const int n = T.n_total;
+    const int nst = B.n_sweep_tiles * 256;
     hipLaunchKernelGGL(k_keys, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.roots,
-                       B.keys[0], B.vals[0]);
+                       B.keys[0], B.vals[0], B.ghist, B.status[0], nst, B.pflag,
+                       B.n_scan_tiles, B.tickets);
+    const int npass = kPasses + (T.nseg > 1 ? 1 : 0);
     int cur = 0;
-    for (int shift = 0; shift < 64; shift += 8) sort_pass(T, B, cur, shift, 0, st);
-    if (T.nseg > 1) {
-        sort_pass(T, B, cur, 0, 1, st);
-        // keep the sorted arrays in buffer 0
-        if (cur != 0) {
-            (void)hipMemcpyAsync(B.keys[0], B.keys[1], sizeof(uint64_t) * n,
-                                 hipMemcpyDeviceToDevice, st);
-            (void)hipMemcpyAsync(B.vals[0], B.vals[1], sizeof(int) * n, hipMemcpyDeviceToDevice,
-                                 st);
-        }
+    for (int p = 0; p < npass; ++p) {
+        const bool last = p == npass - 1;
+        const int mode = p < kPasses ? 0 : 1;
+        hipLaunchKernelGGL(k_sweep, dim3(B.n_sweep_tiles), dim3(kBlock), 0, st, B.keys[cur],
+                           B.vals[cur], B.keys[cur ^ 1], B.vals[cur ^ 1], n, p, mode, T,
+                           B.ghist, B.status[p & 1], last ? nullptr : B.status[(p + 1) & 1],
+                           B.tickets, last ? 1 : 0, x, y, z, mass, B.bodies);
+        cur ^= 1;
     }
-    hipLaunchKernelGGL(k_gather, NBX_GRID(n), B.vals[0], x, y, z, mass, B.bodies, n);
+    if (cur != 0) {  // keep the sorted arrays in buffer 0
+        (void)hipMemcpyAsync(B.keys[0], B.keys[1], sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
+                             st);
+        (void)hipMemcpyAsync(B.vals[0], B.vals[1], sizeof(int) * n, hipMemcpyDeviceToDevice, st);
+    }
+    hipLaunchKernelGGL(k_prefix, dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies, n,
+                       B.prefix, B.pflag, B.pagg, B.pinc, B.tickets, B.ghist);
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
-        hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[0], B.left, B.right, B.par_int,
-                           B.par_leaf, B.plen);
+        hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[0], B.prefix, B.bodies, B.roots,
+                           B.left, B.right, B.plen, B.oct, B.bin_com, B.bin_cell);
 }
 
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
 {
-    const int n = T.n_total, ni = T.ioff[T.nseg];
-    launch_fill_int(B.flags, 0, ni, st);
-    launch_fill_int(B.alloc, 0, kMaxSeg, st);
-    launch_fill_int(B.maxlev, 0, kMaxSeg, st);
-    if (ni > 0) {
-        hipLaunchKernelGGL(k_com, NBX_GRID(n), T, B.bodies, B.par_leaf, B.par_int, B.left,
-                           B.right, B.flags, B.bin_com);
-        hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.par_int, B.plen,
-                           B.cnt, B.first, B.alloc, B.maxlev, B.err);
-        hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni), T, B.roots, B.left, B.right, B.par_int,
-                           B.plen, B.cnt, B.first, B.bin_com, B.bodies, B.rec_a, B.rec_b);
-    }
-    hipLaunchKernelGGL(k_single_roots, dim3(1), dim3(64), 0, st, T, B.bodies, B.rec_a, B.rec_b);
+    const int ni = T.ioff[T.nseg];
+    launch_fill_int(B.alloc, 0, 2 * kMaxSeg, st);  // alloc and maxlev are adjacent
+    if (ni > 0)
+        hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
+                           B.first, B.alloc, B.maxlev, B.err);
+    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.left, B.right,
+                       B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.rec_a, B.rec_b,
+                       B.rec_c);
 }
 
 }  // namespace nbx

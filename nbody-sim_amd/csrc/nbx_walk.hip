@@ -1,13 +1,18 @@
 // nbx_walk.hip -- tree walks on gfx950: gravity, collision impulse, FoF.
 //
-// All three walks share one traversal: a wavefront owns 64 consecutive
-// targets in Morton order (one per lane) and walks the compressed octree
-// with a wave-uniform stack kept in LDS.  A stack entry is a child block
-// {first record, count} plus the 64-bit mask of the lanes whose own walk
-// opened the parent.  Records are read with wave-uniform addresses (scalar
-// loads through the constant cache); each lane evaluates a record only if
-// its bit is set, so every target visits exactly the nodes its own
-// per-body walk (BHTree.cu:555-593) visits, and the wave shares the loads.
+// Gravity (wave-coherent): a wavefront owns 64 consecutive targets in Morton
+// order (one per lane) and walks the compressed octree with a wave-uniform
+// stack in LDS.  A stack entry is a block of child records {first, count}
+// plus the 64-bit mask of the lanes whose own walk opened the parent.  Each
+// iteration pops up to 64 records from the top of the stack, loads them
+// with one lane-parallel vector load into an LDS staging area, and then
+// evaluates them one after the other with LDS broadcasts; a lane evaluates
+// a record only if its bit is set, so every target visits exactly the nodes
+// of its own per-body walk (BHTree.cu:555-593).
+//
+// Collision and FoF (per lane): the search radius is far below the particle
+// spacing, so the per-body paths diverge immediately; each lane descends on
+// its own stack, pruning cells by the exact distance to the cell box.
 #include "nbx_device.hpp"
 #include "nbx_launch.hpp"
 
@@ -15,76 +20,81 @@ namespace nbx {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
-constexpr int kWalkWaves = kBlock / kWave;  // waves per block
+constexpr int kWalkWaves = 1;   // one wavefront per block: walks never synchronise
+constexpr int kWalkBlock = kWave * kWalkWaves;
+constexpr int kGStack = 1024;   // gravity stack entries per wave
+constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
 
 struct StackEntry {
     int first, count;
     unsigned mlo, mhi;
 };
 
-// Visitor interface: visit(a, b, leaf, mine) for one record; returns true
-// when the lane wants the record's children (internal records only).
+struct WaveLds {
+    StackEntry st[kGStack];
+    float4 a[kWave];
+    int4 b[kWave];
+    unsigned long long m[kWave];
+};
+
+// Visitor: visit(a, b, leaf) for one record on the lanes that own it;
+// returns true when the lane opens the (internal) record.
 template <class Visitor>
 __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
-                                 int root, uint64_t active, StackEntry *st, int *err,
-                                 Visitor &vis)
+                                 int root, uint64_t active, WaveLds &L, int *err, Visitor &vis)
 {
     const int lane = __lane_id();
-    int sp = 0;
     // every lane writes the same entry: per-lane program order covers the
     // later reads, no cross-lane ordering is needed
-    st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
-    sp = 1;
+    L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
+    int sp = 1;
     while (sp > 0) {
-        --sp;
-        const StackEntry e = st[sp];
-        const int first = rfl(e.first);
-        const int count = rfl(e.count);
-        const uint64_t mask = ((uint64_t)(unsigned)rfl((int)e.mhi) << 32) |
-                              (uint64_t)(unsigned)rfl((int)e.mlo);
-        const bool mine = (mask >> lane) & 1ull;
-        if (count <= 8) {
-            float4 A[8];
-            int4 Bv[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                A[k] = ra[first + k];
-                Bv[k] = rb[first + k];
+        // take up to 64 records from the top blocks (from the end of a block)
+        int nrec = 0, my_rec = -1;
+        uint64_t my_mask = 0;
+        while (sp > 0 && nrec < kWave) {
+            const StackEntry e = L.st[sp - 1];
+            const int count = rfl(e.count);
+            const int take = min(count, kWave - nrec);
+            if (lane >= nrec && lane < nrec + take) {
+                my_rec = e.first + (count - take) + (lane - nrec);
+                my_mask = ((uint64_t)e.mhi << 32) | (uint64_t)e.mlo;
             }
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k < count) {
-                    const bool leaf = Bv[k].w == 0;
-                    const bool open = mine && vis.visit(A[k], Bv[k], leaf);
-                    const uint64_t om = ballot64(open);
-                    if (om) {
-                        if (sp < kStackCap) {
-                            st[sp] = StackEntry{Bv[k].z, Bv[k].w, (unsigned)om,
-                                                (unsigned)(om >> 32)};
-                            ++sp;
-                        } else if (lane == 0) {
-                            atomicOr(err, kErrStack);
-                        }
-                    }
-                }
+            if (take == count) {
+                --sp;
+            } else {
+                L.st[sp - 1].count = count - take;
             }
-        } else {
-            for (int k = 0; k < count; ++k) {
-                const float4 a = ra[first + k];
-                const int4 b = rb[first + k];
-                const bool leaf = b.w == 0;
-                const bool open = mine && vis.visit(a, b, leaf);
-                const uint64_t om = ballot64(open);
-                if (om) {
-                    if (sp < kStackCap) {
-                        st[sp] = StackEntry{b.z, b.w, (unsigned)om, (unsigned)(om >> 32)};
-                        ++sp;
-                    } else if (lane == 0) {
-                        atomicOr(err, kErrStack);
-                    }
+            nrec += take;
+        }
+        // one vector load per lane, staged in LDS for broadcast reads
+        if (my_rec >= 0) {
+            L.a[lane] = ra[my_rec];
+            L.b[lane] = rb[my_rec];
+            L.m[lane] = my_mask;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int k = 0; k < nrec; ++k) {
+            const float4 a = L.a[k];
+            const int4 b = L.b[k];
+            const uint64_t mk = L.m[k];
+            const bool leaf = b.w == 0;
+            const bool open = ((mk >> lane) & 1ull) && vis.visit(a, b, leaf);
+            const uint64_t om = ballot64(open);
+            if (om) {
+                if (sp < kGStack) {
+                    L.st[sp] = StackEntry{b.z, b.w, (unsigned)om, (unsigned)(om >> 32)};
+                    ++sp;
+                } else if (lane == 0) {
+                    atomicOr(err, kErrStack);
                 }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -113,13 +123,30 @@ struct GravityVisitor {
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_walk(
+// Interaction counts of the same walk (accepted records excluding the own
+// leaf, opened internal records): the algorithmic work of the force pass.
+struct CountVisitor {
+    float bx, by, bz, theta2;
+    int self;
+    int n_acc, n_open;
+    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
+    {
+        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
+        if (open) ++n_open;
+        else if (!(leaf && b.z == self)) ++n_acc;
+        return open;
+    }
+};
+
+__global__ __launch_bounds__(kWalkBlock) void k_walk(
     SegTable T, DiskTable D, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
     float *saz, int *err)
 {
-    __shared__ StackEntry stk[kWalkWaves][kStackCap];
+    __shared__ WaveLds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;  // uniform per wave
@@ -128,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
     GravityVisitor v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), stk[wib], err, v);
+    wave_walk(ra, rb, T.noff[s], ballot64(valid), lds[wib], err, v);
     if (!valid) return;
     const int i = perm[pos];
     float ox, oy, oz;
@@ -156,31 +183,14 @@ __global__ __launch_bounds__(kBlock) void k_walk(
     }
 }
 
-// Interaction counts of the same walk (accepted records excluding the own
-// leaf, opened internal records): the algorithmic work of the force pass.
-struct CountVisitor {
-    float bx, by, bz, theta2;
-    int self;
-    int n_acc, n_open;
-    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
-    {
-        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
-        if (open) ++n_open;
-        else if (!(leaf && b.z == self)) ++n_acc;
-        return open;
-    }
-};
-
-__global__ __launch_bounds__(kBlock) void k_walk_counts(SegTable T, const float4 *__restrict__ ra,
+__global__ __launch_bounds__(kWalkBlock) void k_walk_counts(SegTable T, const float4 *__restrict__ ra,
                                                         const int4 *__restrict__ rb,
                                                         const float4 *__restrict__ bodies,
                                                         const int *__restrict__ perm,
                                                         float theta2, int *n_acc, int *n_open,
                                                         int *err)
 {
-    __shared__ StackEntry stk[kWalkWaves][kStackCap];
+    __shared__ WaveLds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
@@ -189,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_counts(SegTable T, const float4
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
     CountVisitor v{b.x, b.y, b.z, theta2, pos, 0, 0};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), stk[wib], err, v);
+    wave_walk(ra, rb, T.noff[s], ballot64(valid), lds[wib], err, v);
     if (!valid) return;
     const int i = perm[pos];
     n_acc[i] = v.n_acc;
@@ -197,27 +207,97 @@ __global__ __launch_bounds__(kBlock) void k_walk_counts(SegTable T, const float4
 }
 
 // ---------------------------------------------------------------------------
-// Collision (BHTree.cu:620-673) with an exact pair set: a cell is opened
-// when it may hold a body within r_coll (every body of a cell of half-size
-// h lies within 2 sqrt(3) h of its COM); leaves test dist^2 < r_coll^2.
+// Per-lane descent for small search radii.  A cell record {centre, h} holds
+// every body classified into it within h plus the rounding of the float
+// descent that produced its centre: at most 21 roundings of half an ulp of
+// |root centre| + root half size, bounded by `guard` = 2e-6 of that scale;
+// the cell is opened when the target lies within `radius` of that box.
 // ---------------------------------------------------------------------------
-struct CollideVisitor {
-    float bx, by, bz, rc, rc2, e;
+__device__ inline bool near_cell(float tx, float ty, float tz, const float4 &c, float radius,
+                                 float guard)
+{
+    const float slack = c.w + guard;
+    const float ex = fmaxf(fabsf(tx - c.x) - slack, 0.f);
+    const float ey = fmaxf(fabsf(ty - c.y) - slack, 0.f);
+    const float ez = fmaxf(fabsf(tz - c.z) - slack, 0.f);
+    const float r = radius * 1.0001f;
+    return ex * ex + ey * ey + ez * ez <= r * r;
+}
+
+// Pair(p, j_sorted): called for every leaf within the pruned walk.
+__device__ inline float root_guard(const Root &R)
+{
+    return (fabsf(R.c[0]) + fabsf(R.c[1]) + fabsf(R.c[2]) + R.c[3]) * 2e-6f;
+}
+
+template <class Pair>
+__device__ inline void lane_walk(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
+                                 int root, float tx, float ty, float tz, float radius,
+                                 float guard, int2 *st, int *err, Pair &pair)
+{
+    // st: this lane's stack, stride kWave between entries (bank-conflict free)
+    int sp = 0;
+    {
+        const int4 b = rb[root];
+        const float4 c = rc[root];
+        if (b.w == 0) {
+            pair(c, b.z);
+            return;
+        }
+        if (!near_cell(tx, ty, tz, c, radius, guard)) return;
+        st[0] = make_int2(b.z, b.w);
+        sp = 1;
+    }
+    while (sp > 0) {
+        --sp;
+        const int2 e = st[sp * kWave];
+        const int first = e.x, count = e.y;
+        for (int k0 = 0; k0 < count; k0 += 8) {
+            int4 B[8];
+            float4 Cc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                B[k] = rb[first + k0 + k];
+                Cc[k] = rc[first + k0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k0 + k >= count) break;
+                if (B[k].w == 0) {
+                    pair(Cc[k], B[k].z);
+                } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
+                    if (sp < kLStack) {
+                        st[sp * kWave] = make_int2(B[k].z, B[k].w);
+                        ++sp;
+                    } else {
+                        atomicOr(err, kErrStack);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Collision pass 1 (BHTree.cu:628-672) on the exact pair set
+// dist^2 < r_coll^2: J = -(1+e) dvn / 2 for approaching pairs (dvn < 0),
+// dv_i -= J n, accumulated in the walk's (Morton) order.
+struct CollidePair {
+    float bx, by, bz, rc2, e;
     float vix, viy, viz;
     int self;
     const int *perm;
     const float *vx, *vy, *vz;
     float dvx, dvy, dvz;
     int pairs;
-    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
+    __device__ inline void operator()(const float4 &c, int p)
     {
 #pragma clang fp contract(off)
-        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
+        if (p == self) return;
+        const float dx = c.x - bx, dy = c.y - by, dz = c.z - bz;
         const float dist2 = dx * dx + dy * dy + dz * dz;
-        if (!leaf) return sqrtf(dist2) - 3.5f * __int_as_float(b.x) < rc * 1.01f;
-        if (b.z == self || !(dist2 < rc2)) return false;
+        if (!(dist2 < rc2)) return;
         ++pairs;
-        const int j = perm[b.z];
+        const int j = perm[p];
         const float dist = sqrtf(dist2) + 1e-10f;
         const float nx = dx / dist, ny = dy / dist, nz = dz / dist;
         const float dvn = (vx[j] - vix) * nx + (vy[j] - viy) * ny + (vz[j] - viz) * nz;
@@ -227,30 +307,30 @@ struct CollideVisitor {
             dvy += -J * ny;
             dvz += -J * nz;
         }
-        return false;
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_collide(
-    SegTable T, DiskTable D, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
+__global__ __launch_bounds__(kWalkBlock) void k_collide(
+    SegTable T, DiskTable D, const float4 *__restrict__ rc, const int4 *__restrict__ rb,
     const float4 *__restrict__ bodies, const int *__restrict__ perm,
     const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
-    float restitution, float *dv, int *err)
+    const Root *__restrict__ roots, float restitution, float *dv, int *err)
 {
-    __shared__ StackEntry stk[kWalkWaves][kStackCap];
+    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
     const int s = seg_of_wave(T, w);
     const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
-    const bool valid = pos < T.off[s + 1];
-    const float4 b = bodies[valid ? pos : T.off[s]];
-    const int i = perm[valid ? pos : T.off[s]];
-    const float rc = D.d[s].r_coll;
-    CollideVisitor v{b.x, b.y, b.z, rc, rc * rc, restitution, vx[i], vy[i], vz[i], pos, perm,
-                     vx, vy, vz, 0.f, 0.f, 0.f, 0};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), stk[wib], err, v);
-    if (!valid) return;
+    if (pos >= T.off[s + 1]) return;
+    const float4 b = bodies[pos];
+    const int i = perm[pos];
+    const float r = D.d[s].r_coll;
+    CollidePair v{b.x, b.y, b.z, r * r, restitution, vx[i], vy[i], vz[i], pos, perm, vx, vy, vz,
+                  0.f, 0.f, 0.f, 0};
+    if (r > 0.f)
+        lane_walk(rc, rb, T.noff[s], b.x, b.y, b.z, r, root_guard(roots[s]),
+                  st + wib * kLStack * kWave + __lane_id(), err, v);
     const int n = T.n_total;
     dv[i] = v.dvx;
     dv[n + i] = v.dvy;
@@ -259,10 +339,10 @@ __global__ __launch_bounds__(kBlock) void k_collide(
 }
 
 // ---------------------------------------------------------------------------
-// FoF: same neighbour walk with the predicate of Aggregate.cpp:38-40,
-// (dx^2 + dy^2) + dz^2 < ll^2 in float without contraction; pairs are
-// merged by linking the larger root under the smaller one, so each group's
-// root is its smallest index (the canonical label).
+// FoF: the predicate of Aggregate.cpp:38-40, (dx^2 + dy^2) + dz^2 < ll^2 in
+// float without contraction; pairs are merged by linking the larger root
+// under the smaller one, so each group's root is its smallest index (the
+// canonical label).
 // ---------------------------------------------------------------------------
 __device__ inline int uf_find(int *parent, int x)
 {
@@ -296,44 +376,41 @@ __device__ inline void uf_union(int *parent, int a, int b)
     }
 }
 
-struct FofVisitor {
-    float bx, by, bz, ll, ll2;
+struct FofPair {
+    float bx, by, bz, ll2;
     int self_i;
     const int *perm;
     int *parent;
-    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
+    __device__ inline void operator()(const float4 &c, int p)
     {
 #pragma clang fp contract(off)
-        if (!leaf) {
-            const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            return sqrtf(d2) - 3.5f * __int_as_float(b.x) < ll * 1.01f;
-        }
-        const int j = perm[b.z];
-        if (j <= self_i) return false;
+        const int j = perm[p];
+        if (j <= self_i) return;
         // Aggregate.cpp:38 computes p[i]-p[j] with i < j
-        const float dx = bx - a.x, dy = by - a.y, dz = bz - a.z;
+        const float dx = bx - c.x, dy = by - c.y, dz = bz - c.z;
         if (dx * dx + dy * dy + dz * dz < ll2) uf_union(parent, self_i, j);
-        return false;
     }
 };
 
-__global__ __launch_bounds__(kBlock) void k_fof_link(SegTable T, const float4 *__restrict__ ra,
+__global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float4 *__restrict__ rc,
                                                      const int4 *__restrict__ rb,
                                                      const float4 *__restrict__ bodies,
-                                                     const int *__restrict__ perm, float ll,
+                                                     const int *__restrict__ perm,
+                                                     const Root *__restrict__ roots, float ll,
                                                      int *parent, int *err)
 {
-    __shared__ StackEntry stk[kWalkWaves][kStackCap];
+    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
     const int s = seg_of_wave(T, w);
     const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
-    const bool valid = pos < T.off[s + 1];
-    const float4 b = bodies[valid ? pos : T.off[s]];
-    FofVisitor v{b.x, b.y, b.z, ll, ll * ll, perm[valid ? pos : T.off[s]], perm, parent};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), stk[wib], err, v);
+    if (pos >= T.off[s + 1]) return;
+    const float4 b = bodies[pos];
+    FofPair v{b.x, b.y, b.z, ll * ll, perm[pos], perm, parent};
+    if (ll > 0.f)
+        lane_walk(rc, rb, T.noff[s], b.x, b.y, b.z, ll, root_guard(roots[s]),
+                  st + wib * kLStack * kWave + __lane_id(), err, v);
 }
 
 __global__ void k_iota(int *p, int n)
@@ -357,7 +434,7 @@ void launch_walk(const SegTable &T, const DiskTable &D, const WalkArgs &W, Bufs 
                  hipStream_t st)
 {
     const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_walk, dim3(cdiv(waves, kWalkWaves)), dim3(kBlock), 0, st, T, D,
+    hipLaunchKernelGGL(k_walk, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
                        B.rec_a, B.rec_b, B.bodies, B.vals[0], W.theta2, W.eps2, W.softening,
                        W.star_mass, W.ax, W.ay, W.az, W.sax, W.say, W.saz, B.err);
 }
@@ -366,7 +443,7 @@ void launch_walk_counts(const SegTable &T, float theta2, int *n_acc, int *n_open
                         hipStream_t st)
 {
     const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_walk_counts, dim3(cdiv(waves, kWalkWaves)), dim3(kBlock), 0, st, T,
+    hipLaunchKernelGGL(k_walk_counts, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
                        B.rec_a, B.rec_b, B.bodies, B.vals[0], theta2, n_acc, n_open, B.err);
 }
 
@@ -374,17 +451,17 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
                     const float *vz, float restitution, Bufs &B, hipStream_t st)
 {
     const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kBlock), 0, st, T, D,
-                       B.rec_a, B.rec_b, B.bodies, B.vals[0], vx, vy, vz, restitution, B.dv,
-                       B.err);
+    hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
+                       B.rec_c, B.rec_b, B.bodies, B.vals[0], vx, vy, vz, B.roots, restitution,
+                       B.dv, B.err);
 }
 
 void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t st)
 {
     const int n = T.n_total, waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels, n);
-    hipLaunchKernelGGL(k_fof_link, dim3(cdiv(waves, kWalkWaves)), dim3(kBlock), 0, st, T,
-                       B.rec_a, B.rec_b, B.bodies, B.vals[0], ll, B.labels, B.err);
+    hipLaunchKernelGGL(k_fof_link, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
+                       B.rec_c, B.rec_b, B.bodies, B.vals[0], B.roots, ll, B.labels, B.err);
     hipLaunchKernelGGL(k_fof_final, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels,
                        labels, n);
 }
