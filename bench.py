@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU baseline sample length in steps (0: about 15 s)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep leg")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step's kernels directly instead of replaying a hipGraph")
     return ap.parse_args()
 
 
@@ -86,7 +90,7 @@ def main():
     d = make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024)
     r_coll = nbx.particlePhysicalRadius(np.float32(DISK_MASS / n))
     sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
-                         gas_drag_rate=1e-3, collisions=True)
+                         gas_drag_rate=1e-3, collisions=True, use_graph=not args.no_graph)
     pert = nbx.makePerturber(True, 10.0, 5.2, 0.0, 1.0)
     dp = nbx.disk_params(r_coll, pert)
     p = nbx.to_gpu(d)
@@ -149,16 +153,75 @@ def main():
         "config": {"workload": "plain full step, seeded disk N=131072 (3e-5 M_sun, "
                                "r 0.5-4 AU), theta 0.5, eps 1e-3, dt 0.01, collisions, "
                                "drag 1e-3, perturber 10 M_Jup @ 5.2 AU, FoF every 100",
-                   "n": n, "theta": 0.5, "parallelism": "single GPU"},
+                   "n": n, "theta": 0.5, "parallelism": "single GPU",
+                   "launch": "direct" if args.no_graph else "hipGraph replay per step"},
         "roofline": dominant,
         "roofline_walk": roof_walk,
         "phases_ms": phases,
         "force_err": acc_stats,
     }
+    if not args.no_batch:
+        out["batched_sweep"] = batched_sweep(nbx, np, torch, make_disk, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024), r_coll,
                                            10.0, args.cpu_steps)
     print(json.dumps(out))
+
+
+SWEEP_N = [16384, 32768, 65536, 98304, 131072, 262144]
+SWEEP_MJUP = [0.0, 0.3, 1.0, 3.0, 5.0, 10.0]
+
+
+def batched_sweep(nbx, np, torch, make_disk, args):
+    """Six disks of uneven N, one perturber mass each, advanced together by
+    one replayed step (every phase one launch for all disks) against the
+    same six disks stepped one after the other as plain runs."""
+    k = args.batch_steps
+    disks = [make_disk(n, DISK_MASS, 0.5, 4.0, seed=100 + i) for i, n in enumerate(SWEEP_N)]
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True, use_graph=not args.no_graph)
+    dps = []
+    for n, mj in zip(SWEEP_N, SWEEP_MJUP):
+        r_coll = nbx.particlePhysicalRadius(np.float32(DISK_MASS / n))
+        dps.append(nbx.disk_params(r_coll, nbx.makePerturber(mj > 0, mj, 5.2, 0.0, 1.0)))
+    # sequential plain runs
+    plain = []
+    for dd, dp in zip(disks, dps):
+        p = nbx.to_gpu(dd)
+        t = nbx.allocateOctree(p.n)
+        nbx.init_forces(t, p, sp, dp)
+        nbx.set_async(t, True)
+        plain.append((p, t, dp))
+    for p, t, dp in plain:  # warm-up (graph capture)
+        nbx.step(t, p, sp, dp, 0, 5)
+        nbx.sync(t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p, t, dp in plain:
+        nbx.step(t, p, sp, dp, 5, k)
+        nbx.sync(t)
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    # one batched tree over all disks
+    cat = {key: np.concatenate([dd[key] for dd in disks]) for key in disks[0]}
+    pb = nbx.to_gpu(cat)
+    tb = nbx.allocateOctreeBatch(SWEEP_N)
+    nbx.init_forces(tb, pb, sp, dps)
+    nbx.set_async(tb, True)
+    nbx.step(tb, pb, sp, dps, 0, 5)
+    nbx.sync(tb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nbx.step(tb, pb, sp, dps, 5, k)
+    nbx.sync(tb)
+    torch.cuda.synchronize()
+    bat_s = time.perf_counter() - t0
+    total = float(sum(SWEEP_N))
+    return {"disks": SWEEP_N, "perturber_mjup": SWEEP_MJUP, "steps": k,
+            "batched_particle_steps_per_s": total * k / bat_s,
+            "sequential_particle_steps_per_s": total * k / seq_s,
+            "speedup_vs_sequential": seq_s / bat_s,
+            "batched_ms_per_step": bat_s / k * 1e3, "sequential_ms_per_step": seq_s / k * 1e3}
 
 
 def force_accuracy(nbx, t, p, n, n_targets=1024):

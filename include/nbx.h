@@ -78,6 +78,11 @@ int nbx_tree_reset(nbx_tree *t);
 /* Stream control: `stream` is a hipStream_t (NULL = the tree's own stream).
  * async=1 makes entry points return without waiting for the device. */
 int nbx_tree_set_stream(nbx_tree *t, void *stream);
+/* No reference counterpart (accuracy option): multipole order of the tree
+ * walk, 1 = monopole (the reference, BHTree.cu:581-583, default) or 2 = add
+ * each accepted node's traceless quadrupole.  The opening rule and the
+ * accepted set are unchanged.  Invalidates the current tree. */
+int nbx_tree_set_multipole(nbx_tree *t, int order);
 int nbx_tree_set_async(nbx_tree *t, int async);
 int nbx_tree_sync(nbx_tree *t);
 
@@ -155,6 +160,19 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
  * updatePerturber (Perturber.h:60-66) evaluated on the host. */
 int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
              const nbx_disk_params *dp, int32_t step0, int32_t n_steps);
+
+/* ---- precision (no reference counterpart) -------------------------------- */
+/* compensated=1: the KDK position / velocity updates (and the drag) of
+ * nbx_step are evaluated in FP64 on hi + lo, the caller's float arrays hold
+ * the rounded values and the tree holds the low-order parts; forces stay in
+ * FP32.  Enabling clears the low parts (the state is its float values). */
+int nbx_tree_set_precision(nbx_tree *t, int compensated);
+/* FP64 sums over the state (low parts included in compensated mode):
+ * out[0] kinetic energy, out[1] softened star potential, out[2..4] angular
+ * momentum, out[5] softened pairwise self potential (O(n^2), only if
+ * with_self, else 0), out[6] total mass.  out is a host array of 7. */
+int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float softening,
+                    int with_self, double *out);
 
 /* ---- diagnostics --------------------------------------------------------- */
 /* Tree statistics of the last build: number of octree nodes and maximum

@@ -52,9 +52,16 @@ struct nbx_tree {
     int stage = kNone;
     int *h_err = nullptr;  // pinned copy of B.err
     bool timing = false;
-    std::vector<hipEvent_t> ev;  // timing events: pairs per phase
+    std::vector<hipEvent_t> ev;  // timing events: 6 per step
     float phase_ms[5] = {0, 0, 0, 0, 0};
     int phase_n = 0;
+    // per-step disk table staging (pinned) and the event that guards it
+    DiskDev *h_dtab = nullptr;
+    hipEvent_t dtab_done = nullptr;
+    // captured single step: replayed once per step with the device step counter
+    hipGraphExec_t graph = nullptr;
+    std::vector<unsigned char> graph_key;
+    bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
 };
 
 namespace {
@@ -123,6 +130,7 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.pflag, B.n_scan_tiles);
     ALLOC(B.pagg, B.n_scan_tiles);
     ALLOC(B.pinc, B.n_scan_tiles);
+    ALLOC(B.pflag2, B.n_scan_tiles);
     ALLOC(B.bodies, n);
     ALLOC(B.left, n);
     ALLOC(B.right, n);
@@ -143,6 +151,7 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.dv, 3 * n);
     ALLOC(B.err, 4);
     ALLOC(B.labels, n);
+    ALLOC(B.step_ctr, 1);
 #undef ALLOC
     HIPCHK(hipMemset(B.rec_a, 0, sizeof(float4) * B.n_rec));
     HIPCHK(hipMemset(B.rec_b, 0, sizeof(int4) * B.n_rec));
@@ -163,12 +172,20 @@ void free_bufs(nbx_tree *t)
                     B.status[1], B.tickets, B.prefix, B.pflag,  B.pagg,    B.pinc,
                     B.bodies,  B.left,    B.right,   B.plen,    B.oct,     B.cnt,
                     B.first,   B.alloc,   B.bin_com, B.bin_cell, B.rec_a,  B.rec_b,
-                    B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels};
+                    B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels,
+                    B.step_ctr, B.dtab,    B.pflag2,  B.prefix2, B.pagg2,   B.pinc2,
+                    B.bin_q,   B.rec_q,   B.lo,      B.diag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
+    if (t->h_dtab) (void)hipHostFree(t->h_dtab);
+    if (t->dtab_done) (void)hipEventDestroy(t->dtab_done);
+    if (t->graph) (void)hipGraphExecDestroy(t->graph);
     B = Bufs{};
     t->h_err = nullptr;
+    t->h_dtab = nullptr;
+    t->dtab_done = nullptr;
+    t->graph = nullptr;
 }
 
 int check_tree_particles(nbx_tree *t, const nbx_particles *p)
@@ -238,16 +255,46 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
     return NBX_OK;
 }
 
-DiskTable disk_table(const nbx_tree *t, const nbx_disk_params *dp, float sim_time, bool pert)
+// r_coll of every disk (constant over a run): passed by value to the walks.
+DiskTable disk_rcoll(const nbx_tree *t, const nbx_disk_params *dp)
 {
     DiskTable D;
     memset(&D, 0, sizeof(D));
-    for (int s = 0; s < t->T.nseg; ++s) {
-        DiskDev &d = D.d[s];
-        if (!dp) continue;
-        d.r_coll = dp[s].r_coll;
-        if (pert && dp[s].perturber_enabled) {
-            // updatePerturber (Perturber.h:60-66), float arithmetic on the host
+    for (int s = 0; s < t->T.nseg; ++s) D.d[s].r_coll = dp ? dp[s].r_coll : 0.f;
+    return D;
+}
+
+// Per-step perturber positions of steps [step0, step0+n) for every disk,
+// from updatePerturber (Perturber.h:60-66) in float on the host, staged in
+// pinned memory and copied to B.dtab on the tree's stream; the step counter
+// is reset so the walk of step k reads row k.
+int upload_steps(nbx_tree *t, const nbx_disk_params *dp, int step0, int n, float dt, bool t0)
+{
+    Bufs &B = t->B;
+    if (n > B.dtab_cap) {
+        HIPCHK(hipStreamSynchronize(t->stream));
+        if (B.dtab) HIPCHK(hipFree(B.dtab));
+        if (t->h_dtab) HIPCHK(hipHostFree(t->h_dtab));
+        B.dtab = nullptr;
+        t->h_dtab = nullptr;
+        const int cap = n < 64 ? 64 : n;
+        HIPCHK(hipMalloc((void **)&B.dtab, sizeof(DiskDev) * kMaxSeg * cap));
+        HIPCHK(hipHostMalloc((void **)&t->h_dtab, sizeof(DiskDev) * kMaxSeg * cap));
+        B.dtab_cap = cap;
+        if (t->graph) {  // the captured step holds the old table address
+            HIPCHK(hipGraphExecDestroy(t->graph));
+            t->graph = nullptr;
+        }
+    }
+    if (!t->dtab_done) HIPCHK(hipEventCreateWithFlags(&t->dtab_done, hipEventDisableTiming));
+    HIPCHK(hipEventSynchronize(t->dtab_done));  // the previous upload has left the staging
+    memset(t->h_dtab, 0, sizeof(DiskDev) * kMaxSeg * n);
+    for (int k = 0; k < n; ++k) {
+        const float sim_time = t0 ? 0.0f : (float)(step0 + k + 1) * dt;  // main.cu:110
+        for (int s = 0; s < t->T.nseg; ++s) {
+            DiskDev &d = t->h_dtab[k * kMaxSeg + s];
+            d.r_coll = dp[s].r_coll;
+            if (!dp[s].perturber_enabled) continue;
             float phi = dp[s].pert_phase0 + dp[s].pert_omega * sim_time;
             d.px = dp[s].pert_radius_au * cosf(phi);
             d.pz = dp[s].pert_radius_au * sinf(phi);
@@ -255,7 +302,11 @@ DiskTable disk_table(const nbx_tree *t, const nbx_disk_params *dp, float sim_tim
             d.pmass = dp[s].pert_mass_msun;
         }
     }
-    return D;
+    HIPCHK(hipMemcpyAsync(B.dtab, t->h_dtab, sizeof(DiskDev) * kMaxSeg * n,
+                          hipMemcpyHostToDevice, t->stream));
+    HIPCHK(hipEventRecord(t->dtab_done, t->stream));
+    launch_fill_int(B.step_ctr, 0, 1, t->stream);
+    return NBX_OK;
 }
 
 void ev_record(nbx_tree *t, int idx)
@@ -368,6 +419,71 @@ int nbx_tree_set_stream(nbx_tree *t, void *stream)
     return NBX_OK;
 }
 
+int nbx_tree_set_multipole(nbx_tree *t, int order)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (order != 1 && order != 2) return fail(NBX_E_ARG, "multipole order %d not in {1, 2}", order);
+    Bufs &B = t->B;
+    if (order == 2 && !B.prefix2) {
+        const int n = t->T.n_total;
+        int rc;
+        if ((rc = dmalloc(&B.prefix2, (size_t)n + 1)) != NBX_OK) return rc;
+        if ((rc = dmalloc(&B.pagg2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
+        if ((rc = dmalloc(&B.pinc2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
+        if ((rc = dmalloc(&B.bin_q, (size_t)2 * n)) != NBX_OK) return rc;
+        if ((rc = dmalloc(&B.rec_q, (size_t)2 * B.n_rec)) != NBX_OK) return rc;
+        HIPCHK(hipMemset(B.rec_q, 0, sizeof(float4) * 2 * B.n_rec));
+    }
+    if (B.quad != (order == 2)) {
+        B.quad = order == 2;
+        t->stage = kNone;  // records must be rebuilt with (or without) moments
+        if (t->graph) {
+            HIPCHK(hipGraphExecDestroy(t->graph));
+            t->graph = nullptr;
+        }
+    }
+    return NBX_OK;
+}
+
+int nbx_tree_set_precision(nbx_tree *t, int compensated)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    Bufs &B = t->B;
+    const int n = t->T.n_total;
+    if (compensated) {
+        if (!B.lo) {
+            int rc = dmalloc(&B.lo, (size_t)6 * n);
+            if (rc) return rc;
+        }
+        HIPCHK(hipStreamSynchronize(t->stream));
+        HIPCHK(hipMemset(B.lo, 0, sizeof(float) * 6 * n));  // state = its float values
+    }
+    if (t->compensated != (compensated != 0) && t->graph) {
+        HIPCHK(hipGraphExecDestroy(t->graph));
+        t->graph = nullptr;
+    }
+    t->compensated = compensated != 0;
+    return NBX_OK;
+}
+
+int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float softening,
+                    int with_self, double *out)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!out) return fail(NBX_E_ARG, "null output");
+    if (!t->B.diag) {
+        rc = dmalloc(&t->B.diag, 8);
+        if (rc) return rc;
+    }
+    launch_diagnostics(p->x, p->y, p->z, p->vx, p->vy, p->vz, p->mass,
+                       t->compensated ? t->B.lo : nullptr, p->n, star_mass, softening, with_self,
+                       t->B, t->stream);
+    HIPCHK(hipMemcpyAsync(out, t->B.diag, 7 * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    return finish(t);
+}
+
 int nbx_tree_set_async(nbx_tree *t, int async)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
@@ -426,7 +542,7 @@ int nbx_sort_idx(nbx_tree *t, int32_t *dev_out)
 {
     if (!t || !dev_out) return fail(NBX_E_ARG, "null argument");
     if (t->stage < kBuilt) return fail(NBX_E_STATE, "sort_idx before build");
-    HIPCHK(hipMemcpyAsync(dev_out, t->B.vals[0], sizeof(int32_t) * t->T.n_total,
+    HIPCHK(hipMemcpyAsync(dev_out, t->B.vals[t->B.cur], sizeof(int32_t) * t->T.n_total,
                           hipMemcpyDeviceToDevice, t->stream));
     return finish(t);
 }
@@ -435,7 +551,7 @@ int nbx_sorted_keys(nbx_tree *t, uint64_t *dev_out)
 {
     if (!t || !dev_out) return fail(NBX_E_ARG, "null argument");
     if (t->stage < kBuilt) return fail(NBX_E_STATE, "keys before build");
-    HIPCHK(hipMemcpyAsync(dev_out, t->B.keys[0], sizeof(uint64_t) * t->T.n_total,
+    HIPCHK(hipMemcpyAsync(dev_out, t->B.keys[t->B.cur], sizeof(uint64_t) * t->T.n_total,
                           hipMemcpyDeviceToDevice, t->stream));
     return finish(t);
 }
@@ -447,10 +563,9 @@ int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, fl
     if (rc) return rc;
     if (t->stage < kSummarized) return fail(NBX_E_STATE, "launchForceKernel before summarize");
     if (!(theta > 0.f)) return fail(NBX_E_ARG, "theta must be positive");
-    WalkArgs W{theta * theta, softening * softening, softening, star_mass, p->ax, p->ay,
+    WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad, p->ax, p->ay,
                p->az, self_ax, self_ay, self_az};
-    DiskTable D = disk_table(t, nullptr, 0.f, false);
-    launch_walk(t->T, D, W, t->B, t->stream);
+    launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);
     return finish(t);
 }
 
@@ -478,7 +593,7 @@ int nbx_collide(nbx_tree *t, nbx_particles *p, float r_coll, float restitution)
     launch_collide(t->T, D, p->vx, p->vy, p->vz, restitution, t->B, t->stream);
     // pass 2 (BHTree.cu:676-686): v += dv
     launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az, t->B.dv, p->x, p->z, 0.f, 0, 1.f,
-                     0.f, p->n, t->stream);
+                     0.f, p->n, nullptr, t->stream);
     return finish(t);
 }
 
@@ -547,10 +662,65 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
     WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
-    DiskTable D = disk_table(t, dp, 0.0f, true);
-    launch_walk(t->T, D, W, t->B, t->stream);
+               sp->star_mass, t->B.quad, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+    rc = upload_steps(t, dp, 0, 1, sp->dt, true);
+    if (rc) return rc;
+    launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
     return finish(t);
+}
+
+static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                         const DiskTable &D, int e0)
+{
+    const float dt = sp->dt, dth = sp->dt * 0.5f;
+    const int has_drag = sp->gas_drag_rate > 0.0f;
+    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
+    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
+               sp->star_mass, t->B.quad, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+    if (e0 >= 0) ev_record(t, e0 + 0);
+    if (t->compensated)
+        launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
+                                 p->az, dth, dt, t->B, t->stream);
+    else
+        launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
+                               p->az, dth, dt, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 1);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_summarize(t->T, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 2);
+    launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 3);
+    if (sp->collisions)
+        launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 4);
+    if (t->compensated)
+        launch_kick_drag_c(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
+                           sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag,
+                           has_drag ? exp(-(double)sp->gas_drag_rate * (double)sp->dt) : 1.0,
+                           sp->star_mass, p->n, t->B.step_ctr, t->B, t->stream);
+    else
+        launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
+                         sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag, decay,
+                         sp->star_mass, p->n, t->B.step_ctr, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 5);
+}
+
+// Everything a captured step bakes into its kernel arguments.
+static std::vector<unsigned char> step_key(const nbx_particles *p, const nbx_step_params *sp,
+                                           const DiskTable &D, const Bufs &B)
+{
+    std::vector<unsigned char> k(sizeof(*p) + sizeof(*sp) + sizeof(D) + sizeof(void *) + 1);
+    unsigned char *q = k.data();
+    memcpy(q, p, sizeof(*p));
+    q += sizeof(*p);
+    memcpy(q, sp, sizeof(*sp));
+    q += sizeof(*sp);
+    memcpy(q, &D, sizeof(D));
+    q += sizeof(D);
+    memcpy(q, &B.dtab, sizeof(void *));
+    q += sizeof(void *);
+    *q = B.lo != nullptr;
+    return k;
 }
 
 int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
@@ -560,51 +730,51 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     if (rc) return rc;
     if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
     if (n_steps < 0 || step0 < 0) return fail(NBX_E_ARG, "negative step range");
-    const int nev = 6 * n_steps;
-    if (t->timing && (int)t->ev.size() < nev) {
-        const size_t old = t->ev.size();
-        t->ev.resize(nev);
-        for (size_t q = old; q < t->ev.size(); ++q) HIPCHK(hipEventCreate(&t->ev[q]));
-    }
-    const float dt = sp->dt, dth = sp->dt * 0.5f;
-    const int has_drag = sp->gas_drag_rate > 0.0f;
-    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;
-    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
-    for (int k = 0; k < n_steps; ++k) {
-        const int step = step0 + k;
-        const float sim_time = (float)(step + 1) * dt;  // main.cu:110
-        const DiskTable D = disk_table(t, dp, sim_time, true);
-        const int e0 = 6 * k;
-        ev_record(t, e0 + 0);
-        launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
-                               p->az, dth, dt, t->B, t->stream);
-        ev_record(t, e0 + 1);
-        launch_root(t->T, t->B, t->stream);
-        launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
-        launch_summarize(t->T, t->B, t->stream);
-        ev_record(t, e0 + 2);
-        launch_walk(t->T, D, W, t->B, t->stream);
-        ev_record(t, e0 + 3);
-        if (sp->collisions)
-            launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
-        ev_record(t, e0 + 4);
-        launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
-                         sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag, decay,
-                         sp->star_mass, p->n, t->stream);
-        ev_record(t, e0 + 5);
-    }
-    if (t->timing && n_steps > 0) {
-        float acc_ms[5] = {0, 0, 0, 0, 0};
-        HIPCHK(hipEventSynchronize(t->ev[nev - 1]));
-        for (int k = 0; k < n_steps; ++k)
-            for (int q = 0; q < 5; ++q) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, t->ev[6 * k + q], t->ev[6 * k + q + 1]));
-                acc_ms[q] += ms;
+    if (n_steps == 0) return finish(t);
+    rc = upload_steps(t, dp, step0, n_steps, sp->dt, false);
+    if (rc) return rc;
+    const DiskTable D = disk_rcoll(t, dp);
+    const bool graph = sp->use_graph && !t->timing;
+    if (graph) {
+        std::vector<unsigned char> key = step_key(p, sp, D, t->B);
+        if (!t->graph || key != t->graph_key) {
+            if (t->graph) {
+                HIPCHK(hipGraphExecDestroy(t->graph));
+                t->graph = nullptr;
             }
-        for (int q = 0; q < 5; ++q) t->phase_ms[q] = acc_ms[q] / n_steps;
-        t->phase_n = n_steps;
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_step(t, p, sp, D, -1);
+            HIPCHK(hipStreamEndCapture(t->stream, &g));
+            hipError_t e = hipGraphInstantiate(&t->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                t->graph = nullptr;
+                return fail(NBX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            }
+            t->graph_key = key;
+        }
+        for (int k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(t->graph, t->stream));
+    } else {
+        const int nev = 6 * n_steps;
+        if (t->timing && (int)t->ev.size() < nev) {
+            const size_t old = t->ev.size();
+            t->ev.resize(nev);
+            for (size_t q = old; q < t->ev.size(); ++q) HIPCHK(hipEventCreate(&t->ev[q]));
+        }
+        for (int k = 0; k < n_steps; ++k) enqueue_step(t, p, sp, D, t->timing ? 6 * k : -1);
+        if (t->timing) {
+            float acc_ms[5] = {0, 0, 0, 0, 0};
+            HIPCHK(hipEventSynchronize(t->ev[nev - 1]));
+            for (int k = 0; k < n_steps; ++k)
+                for (int q = 0; q < 5; ++q) {
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, t->ev[6 * k + q], t->ev[6 * k + q + 1]));
+                    acc_ms[q] += ms;
+                }
+            for (int q = 0; q < 5; ++q) t->phase_ms[q] = acc_ms[q] / n_steps;
+            t->phase_n = n_steps;
+        }
     }
     t->stage = kSummarized;
     return finish(t);

@@ -5,6 +5,12 @@
 
 namespace nbx {
 
+// N doubles (moment vectors of the prefix scans).
+template <int N>
+struct DVec {
+    double v[N];
+};
+
 // Device scratch owned by one nbx_tree.  Sizes are for n bodies over all
 // disks; see nbx_internal.hpp for the layout.
 struct Bufs {
@@ -16,10 +22,16 @@ struct Bufs {
     int *ghist = nullptr;        // [kPasses * 256] global digit histograms
     unsigned *status[2] = {};    // [n_sweep_tiles * 256] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
-    double4 *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
+    DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
     int *pflag = nullptr;        // [n_scan_tiles] scan look-back flags
-    double4 *pagg = nullptr;     // [n_scan_tiles] tile aggregates
-    double4 *pinc = nullptr;     // [n_scan_tiles] inclusive tile prefixes
+    DVec<4> *pagg = nullptr;     // [n_scan_tiles] tile aggregates
+    DVec<4> *pinc = nullptr;     // [n_scan_tiles] inclusive tile prefixes
+    int quad = 0;                // 1: quadrupole moments are built and used
+    DVec<6> *prefix2 = nullptr;  // [n + 1] exclusive prefix of the second moments
+    int *pflag2 = nullptr;
+    DVec<6> *pagg2 = nullptr, *pinc2 = nullptr;
+    float4 *bin_q = nullptr;     // [2 n] traceless quadrupole of binary nodes
+    float4 *rec_q = nullptr;     // [2 n_rec] quadrupole of walk records (0 for leaves)
     float4 *bodies = nullptr;    // [n] sorted {x,y,z,m}
     int *left = nullptr, *right = nullptr;   // [n] binary children
     int *plen = nullptr;         // [n] common-prefix length of internal node
@@ -39,6 +51,12 @@ struct Bufs {
     float *dv = nullptr;         // [3 n] collision velocity deltas
     int *err = nullptr;          // [4] error flags / counters
     int *labels = nullptr;       // [n] FoF scratch
+    int cur = 0;                 // buffer of keys[] / vals[] holding the sorted order
+    float *lo = nullptr;         // [6 n] low-order parts of x,y,z,vx,vy,vz (compensated mode)
+    double *diag = nullptr;      // [8] FP64 diagnostic sums
+    DiskDev *dtab = nullptr;     // [dtab_cap * kMaxSeg] per-step disk values
+    int dtab_cap = 0;
+    int *step_ctr = nullptr;     // [1] step index into dtab, advanced by k_kick_drag
 };
 
 constexpr int kPasses = 8;       // 8-bit digits over the 63-bit key
@@ -49,7 +67,8 @@ constexpr int kScanTile = kBlock * kScanItems;
 // tickets[] slots
 constexpr int kTicketPass = 0;   // + pass (9 slots incl. the disk pass)
 constexpr int kTicketScan = 10;
-constexpr int kBlockCtr = 11;    // last-block counter of the bbox reduction
+constexpr int kTicketScan2 = 12;
+constexpr int kBlockCtr = 15;    // last-block counter of the bbox reduction
 
 
 // ---- integrator (reference-identical float ops) --------------------------
@@ -70,10 +89,26 @@ void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, flo
                             float *vy, float *vz, const float *ax, const float *ay,
                             const float *az, float dt_half, float dt, Bufs &B, hipStream_t st);
 // v += dv (if dv); v += a*dt_half; gas drag (if decay < 1 or has_drag).
+// step_ctr (optional) is advanced by one after the kernel.
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
                       const float *az, const float *dv, const float *x, const float *z,
                       float dt_half, int has_drag, float decay, float star_mass, int n,
-                      hipStream_t st);
+                      int *step_ctr, hipStream_t st);
+
+// Compensated variants: the float state holds round(X), lo[] holds
+// X - round(X); updates are evaluated in FP64 on X = hi + lo.
+void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,
+                              float *vy, float *vz, const float *ax, const float *ay,
+                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st);
+void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const float *ay,
+                        const float *az, const float *dv, const float *x, const float *z,
+                        float dt_half, int has_drag, double decay, float star_mass, int n,
+                        int *step_ctr, Bufs &B, hipStream_t st);
+// FP64 sums: out[0] kinetic, [1] star potential, [2] L_x, [3] L_y, [4] L_z,
+// [5] self potential (pairwise, softened; only if with_self), [6] total mass.
+void launch_diagnostics(const float *x, const float *y, const float *z, const float *vx,
+                        const float *vy, const float *vz, const float *m, const float *lo, int n,
+                        double star_mass, double soft, int with_self, Bufs &B, hipStream_t st);
 
 // ---- tree -----------------------------------------------------------------
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
@@ -88,11 +123,14 @@ void launch_fill_float(float *p, float v, int n, hipStream_t st);
 // ---- walks ----------------------------------------------------------------
 struct WalkArgs {
     float theta2, eps2, softening, star_mass;
+    int quad;                     // 1: add the quadrupole of accepted internal records
     float *ax, *ay, *az;          // original order, written with '='
     float *sax, *say, *saz;       // optional self-gravity outputs
 };
-void launch_walk(const SegTable &T, const DiskTable &D, const WalkArgs &W, Bufs &B,
-                 hipStream_t st);
+// Perturber values per disk come from dtab[*step_ctr][disk] when dtab is
+// non-null (graph-replayable), otherwise no perturber.
+void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
+                 Bufs &B, hipStream_t st);
 // Per-body accepted / opened record counts of the gravity walk.
 void launch_walk_counts(const SegTable &T, float theta2, int *n_acc, int *n_open, Bufs &B,
                         hipStream_t st);

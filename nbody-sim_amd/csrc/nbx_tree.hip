@@ -118,9 +118,10 @@ __global__ void k_kick_drag(float *vx, float *vy, float *vz, const float *__rest
                             const float *__restrict__ ay, const float *__restrict__ az,
                             const float *__restrict__ dv, const float *__restrict__ x,
                             const float *__restrict__ z, float dt_half, int has_drag,
-                            float decay, float star_mass, int n)
+                            float decay, float star_mass, int n, int *step_ctr)
 {
 #pragma clang fp contract(off)
+    if (step_ctr && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step_ctr, 1);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float a = vx[i], b = vy[i], c = vz[i];
@@ -138,6 +139,195 @@ __global__ void k_kick_drag(float *vx, float *vy, float *vz, const float *__rest
     vx[i] = a;
     vy[i] = b;
     vz[i] = c;
+}
+
+// ---------------------------------------------------------------------------
+// Compensated mode (no reference counterpart): the same KDK / drag sequence
+// evaluated in FP64 on X = hi + lo, storing hi = round-to-float(X) in the
+// caller's state and lo = X - hi in the tree.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_kick_drift_bbox_c(
+    SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
+    const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
+    float dt_half, float dt, float *lo, float *partial, Root *roots, int *tickets)
+{
+    const int b = blockIdx.x;
+    const int s = seg_of_block(T, b);
+    const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
+    const int n = T.n_total;
+    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
+    float hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+    if (i < T.off[s + 1]) {
+        float *xs[3] = {x, y, z};
+        float *vs[3] = {vx, vy, vz};
+        const float *as[3] = {ax, ay, az};
+        float o[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double V = (double)vs[k][i] + (double)lo[(3 + k) * n + i];
+            V += (double)as[k][i] * (double)dt_half;
+            double X = (double)xs[k][i] + (double)lo[k * n + i];
+            X += V * (double)dt;
+            const float xh = (float)X, vh = (float)V;
+            xs[k][i] = xh;
+            vs[k][i] = vh;
+            lo[k * n + i] = (float)(X - (double)xh);
+            lo[(3 + k) * n + i] = (float)(V - (double)vh);
+            o[k] = xh;
+        }
+        lx = hx = o[0];
+        ly = hy = o[1];
+        lz = hz = o[2];
+    }
+    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, nullptr);
+}
+
+__device__ inline void gas_drag_one_d(double xi, double zi, double &vxi, double &vyi,
+                                      double &vzi, double decay, double star_mass)
+{
+    const double r_xz2 = xi * xi + zi * zi;
+    if (r_xz2 < 1e-12) return;
+    const double inv_r = 1.0 / sqrt(r_xz2);
+    const double v_r = (vxi * xi + vzi * zi) * inv_r;
+    const double v_phi = (vxi * (-zi) + vzi * xi) * inv_r;
+    const double v_kep_mag = sqrt(star_mass * inv_r);
+    const double v_kep = (v_phi >= 0.0) ? v_kep_mag : -v_kep_mag;
+    const double v_r_new = v_r * decay;
+    const double v_phi_new = v_kep + (v_phi - v_kep) * decay;
+    vxi = v_r_new * (xi * inv_r) + v_phi_new * (-zi * inv_r);
+    vzi = v_r_new * (zi * inv_r) + v_phi_new * (xi * inv_r);
+    vyi = vyi * decay;
+}
+
+__global__ void k_kick_drag_c(float *vx, float *vy, float *vz, const float *__restrict__ ax,
+                              const float *__restrict__ ay, const float *__restrict__ az,
+                              const float *__restrict__ dv, const float *__restrict__ x,
+                              const float *__restrict__ z, float dt_half, int has_drag,
+                              double decay, float star_mass, int n, int *step_ctr, float *lo)
+{
+    if (step_ctr && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step_ctr, 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double V[3] = {(double)vx[i] + lo[3 * n + i], (double)vy[i] + lo[4 * n + i],
+                   (double)vz[i] + lo[5 * n + i]};
+    const float *as[3] = {ax, ay, az};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (dv) V[k] += (double)dv[k * n + i];
+        V[k] += (double)as[k][i] * (double)dt_half;
+    }
+    if (has_drag)
+        gas_drag_one_d((double)x[i] + lo[i], (double)z[i] + lo[2 * n + i], V[0], V[1], V[2],
+                       decay, (double)star_mass);
+    float *vs[3] = {vx, vy, vz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float h = (float)V[k];
+        vs[k][i] = h;
+        lo[(3 + k) * n + i] = (float)(V[k] - (double)h);
+    }
+}
+
+// FP64 diagnostics: block partial sums then one double atomic per block.
+__global__ __launch_bounds__(kBlock) void k_diag(const float *__restrict__ x,
+                                                 const float *__restrict__ y,
+                                                 const float *__restrict__ z,
+                                                 const float *__restrict__ vx,
+                                                 const float *__restrict__ vy,
+                                                 const float *__restrict__ vz,
+                                                 const float *__restrict__ m,
+                                                 const float *__restrict__ lo, int n,
+                                                 double star_mass, double soft, double *out)
+{
+    __shared__ double s[kBlock / kWave][6];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        double X = x[i], Y = y[i], Z = z[i], U = vx[i], V = vy[i], W = vz[i];
+        if (lo) {
+            X += lo[i]; Y += lo[n + i]; Z += lo[2 * n + i];
+            U += lo[3 * n + i]; V += lo[4 * n + i]; W += lo[5 * n + i];
+        }
+        const double mm = m[i];
+        e[0] = 0.5 * mm * (U * U + V * V + W * W);
+        e[1] = -mm * star_mass / sqrt(X * X + Y * Y + Z * Z + soft * soft);
+        e[2] = mm * (Y * W - Z * V);
+        e[3] = mm * (Z * U - X * W);
+        e[4] = mm * (X * V - Y * U);
+        e[5] = mm;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) e[k] += __shfl_xor(e[k], o, 64);
+    if (__lane_id() == 0)
+        for (int k = 0; k < 6; ++k) s[threadIdx.x >> 6][k] = e[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += s[w][threadIdx.x];
+        const int slot[6] = {0, 1, 2, 3, 4, 6};
+        atomicAdd(&out[slot[threadIdx.x]], v);
+    }
+}
+
+// Pairwise softened potential -1/2 sum_{i != j} m_i m_j / sqrt(r^2 + eps^2) in
+// FP64, sources staged through LDS in tiles of kBlock.
+__global__ __launch_bounds__(kBlock) void k_diag_self(const float *__restrict__ x,
+                                                      const float *__restrict__ y,
+                                                      const float *__restrict__ z,
+                                                      const float *__restrict__ m,
+                                                      const float *__restrict__ lo, int n,
+                                                      double soft, double *out)
+{
+    __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
+    __shared__ double red[kBlock / kWave];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double X = 0, Y = 0, Z = 0, M = 0;
+    if (i < n) {
+        X = (double)x[i] + (lo ? lo[i] : 0.0);
+        Y = (double)y[i] + (lo ? lo[n + i] : 0.0);
+        Z = (double)z[i] + (lo ? lo[2 * n + i] : 0.0);
+        M = m[i];
+    }
+    const double e2 = soft * soft;
+    double phi = 0;
+    for (int base = 0; base < n; base += kBlock) {
+        const int j = base + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = (double)x[j] + (lo ? lo[j] : 0.0);
+            sy[threadIdx.x] = (double)y[j] + (lo ? lo[n + j] : 0.0);
+            sz[threadIdx.x] = (double)z[j] + (lo ? lo[2 * n + j] : 0.0);
+            sm[threadIdx.x] = m[j];
+        } else {
+            sx[threadIdx.x] = sy[threadIdx.x] = sz[threadIdx.x] = 0;
+            sm[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        const int lim = min(kBlock, n - base);
+        for (int k = 0; k < lim; ++k) {
+            if (base + k == i) continue;
+            const double dx = sx[k] - X, dy = sy[k] - Y, dz = sz[k] - Z;
+            phi += sm[k] / sqrt(dx * dx + dy * dy + dz * dz + e2);
+        }
+        __syncthreads();
+    }
+    double v = i < n ? -0.5 * M * phi : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+        atomicAdd(&out[5], t);
+    }
+}
+
+__global__ void k_fill_double(double *p, double v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_fill_int(int *p, int v, int n)
@@ -241,15 +431,18 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
                                                  const Root *__restrict__ roots,
                                                  uint64_t *keys, int *vals, int *ghist,
                                                  unsigned *status0, int n_status, int *pflag,
-                                                 int n_pflag, int *tickets)
+                                                 int n_pflag, int *pflag2, int *tickets)
 {
 #pragma clang fp contract(off)
     __shared__ int h[kPasses * 256];
     for (int q = threadIdx.x; q < kPasses * 256; q += kBlock) h[q] = 0;
     const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
-    for (int q = gid; q < n_pflag; q += gsz) pflag[q] = 0;
-    if (gid <= kTicketScan) tickets[gid] = 0;  // tickets of the radix passes and the scan
+    for (int q = gid; q < n_pflag; q += gsz) {
+        pflag[q] = 0;
+        pflag2[q] = 0;
+    }
+    if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
     __syncthreads();
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
@@ -423,65 +616,97 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
 }
 
 // ===========================================================================
-// Exclusive prefix P[i] = sum_{s<i} {m, m x, m y, m z} of the sorted bodies
-// in double precision (single pass, decoupled look-back).  Node mass and
-// COM follow as differences over the node's range of sorted positions.
+// Exclusive prefix P[i] = sum_{s<i} e(s) of per-body moment vectors of the
+// sorted bodies in double precision (single pass, decoupled look-back):
+//   KIND 0: e = {m, m x, m y, m z}            (mass and COM of every node)
+//   KIND 1: e = {m xx, m yy, m zz, m xy, m xz, m yz}   (quadrupoles)
+// A node's moments are differences over its range of sorted positions.
 // ===========================================================================
-__device__ inline double4 d4add(double4 a, double4 b)
+template <int N>
+__device__ inline DVec<N> dv_add(const DVec<N> &a, const DVec<N> &b)
 {
-    return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+    DVec<N> r;
+#pragma unroll
+    for (int q = 0; q < N; ++q) r.v[q] = a.v[q] + b.v[q];
+    return r;
 }
 
-__device__ inline double4 d4shfl_up(double4 v, int o)
+template <int N>
+__device__ inline DVec<N> dv_zero()
 {
-    return make_double4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64),
-                        __shfl_up(v.w, o, 64));
+    DVec<N> r;
+#pragma unroll
+    for (int q = 0; q < N; ++q) r.v[q] = 0.0;
+    return r;
 }
 
+template <int N>
+__device__ inline DVec<N> dv_shfl_up(const DVec<N> &a, int o)
+{
+    DVec<N> r;
+#pragma unroll
+    for (int q = 0; q < N; ++q) r.v[q] = __shfl_up(a.v[q], o, 64);
+    return r;
+}
+
+template <int N, int KIND>
 __global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bodies, int n,
-                                                   double4 *P, int *pflag, double4 *pagg,
-                                                   double4 *pinc, int *tickets, int *ghist)
+                                                   DVec<N> *P, int *pflag, DVec<N> *pagg,
+                                                   DVec<N> *pinc, int *tickets, int slot,
+                                                   int *ghist)
 {
     __shared__ int s_tile;
-    __shared__ double4 s_w[kBlock / kWave];
-    __shared__ double4 s_pre;
+    __shared__ DVec<N> s_w[kBlock / kWave];
+    __shared__ DVec<N> s_pre;
     const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
-    if (t == 0) s_tile = atomicAdd(&tickets[kTicketScan], 1);
+    if (t == 0) s_tile = atomicAdd(&tickets[slot], 1);
     __syncthreads();
     const int tile = s_tile;
     const int base = tile * kScanTile + t * kScanItems;
-    double4 v[kScanItems];
-    double4 sum = make_double4(0, 0, 0, 0);
+    DVec<N> v[kScanItems];
+    DVec<N> sum = dv_zero<N>();
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int i = base + k;
-        double4 e = make_double4(0, 0, 0, 0);
+        DVec<N> e = dv_zero<N>();
         if (i < n) {
             const float4 b = bodies[i];
-            const double mm = b.w;
-            e = make_double4(mm, mm * b.x, mm * b.y, mm * b.z);
+            const double mm = b.w, x = b.x, y = b.y, z = b.z;
+            if (KIND == 0) {
+                e.v[0] = mm;
+                e.v[1] = mm * x;
+                e.v[2] = mm * y;
+                e.v[3] = mm * z;
+            } else {
+                e.v[0] = mm * x * x;
+                e.v[1] = mm * y * y;
+                e.v[2] = mm * z * z;
+                e.v[3] = mm * x * y;
+                e.v[4] = mm * x * z;
+                e.v[5] = mm * y * z;
+            }
         }
         v[k] = e;
-        sum = d4add(sum, e);
+        sum = dv_add(sum, e);
     }
-    double4 inc = sum;
+    DVec<N> inc = sum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const double4 y = d4shfl_up(inc, o);
-        if (lane >= o) inc = d4add(inc, y);
+        const DVec<N> y = dv_shfl_up(inc, o);
+        if (lane >= o) inc = dv_add(inc, y);
     }
-    double4 excl = d4shfl_up(inc, 1);
-    if (lane == 0) excl = make_double4(0, 0, 0, 0);
+    DVec<N> excl = dv_shfl_up(inc, 1);
+    if (lane == 0) excl = dv_zero<N>();
     if (lane == 63) s_w[w] = inc;
     __syncthreads();
-    double4 before = make_double4(0, 0, 0, 0), agg = make_double4(0, 0, 0, 0);
+    DVec<N> before = dv_zero<N>(), agg = dv_zero<N>();
 #pragma unroll
     for (int k = 0; k < kBlock / kWave; ++k) {
-        if (k < w) before = d4add(before, s_w[k]);
-        agg = d4add(agg, s_w[k]);
+        if (k < w) before = dv_add(before, s_w[k]);
+        agg = dv_add(agg, s_w[k]);
     }
     if (t == 0) {
-        double4 pre = make_double4(0, 0, 0, 0);
+        DVec<N> pre = dv_zero<N>();
         if (tile == 0) {
             pinc[0] = agg;
             __hip_atomic_store(&pflag[0], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -497,25 +722,25 @@ __global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bo
                     continue;
                 }
                 if (f == 2) {
-                    pre = d4add(pre, pinc[j]);
+                    pre = dv_add(pre, pinc[j]);
                     break;
                 }
-                pre = d4add(pre, pagg[j]);
+                pre = dv_add(pre, pagg[j]);
                 --j;
             }
-            pinc[tile] = d4add(pre, agg);
+            pinc[tile] = dv_add(pre, agg);
             __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         s_pre = pre;
-        if (tile == 0) P[0] = make_double4(0, 0, 0, 0);
+        if (tile == 0) P[0] = dv_zero<N>();
     }
-    if (tile == 0)  // histograms are consumed: clear them for the next build
+    if (KIND == 0 && tile == 0)  // histograms are consumed: clear them for the next build
         for (int q = t; q < kPasses * 256; q += kBlock) ghist[q] = 0;
     __syncthreads();
-    double4 run = d4add(s_pre, d4add(before, excl));
+    DVec<N> run = dv_add(s_pre, dv_add(before, excl));
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        run = d4add(run, v[k]);
+        run = dv_add(run, v[k]);
         if (base + k < n) P[base + k + 1] = run;
     }
 }
@@ -544,9 +769,10 @@ __device__ inline int lcp(const uint64_t *__restrict__ k, int n, int a, int b)
 __device__ inline int oct_level(int plen) { return plen >= 64 ? kLevels : (plen - 1) / 3; }
 
 __global__ __launch_bounds__(kBlock) void k_karras(
-    SegTable T, const uint64_t *__restrict__ keys, const double4 *__restrict__ P,
-    const float4 *__restrict__ bodies, const Root *__restrict__ roots, int *left, int *right,
-    int *plen, int *octf, float4 *bin_com, float4 *bin_cell)
+    SegTable T, const uint64_t *__restrict__ keys, const DVec<4> *__restrict__ P,
+    const DVec<6> *__restrict__ P2, const float4 *__restrict__ bodies,
+    const Root *__restrict__ roots, int *left, int *right, int *plen, int *octf,
+    float4 *bin_com, float4 *bin_cell, float4 *bin_q)
 {
 #pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -578,17 +804,35 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     const int dpar = max(lcp(k, n, lo - 1, lo), lcp(k, n, hi, hi + 1));
     octf[j] = (dpar < 0 || oct_level(dpar) != lev || lev == kLevels) ? 1 : 0;
     // mass and COM over sorted positions [lo, hi]
-    const double4 a = P[T.off[s] + lo], b = P[T.off[s] + hi + 1];
-    const double mm = b.x - a.x;
+    const DVec<4> a = P[T.off[s] + lo], b = P[T.off[s] + hi + 1];
+    const double mm = b.v[0] - a.v[0];
+    double cxd = 0, cyd = 0, czd = 0;
     float4 c;
     if (mm > 0.0) {
-        c = make_float4((float)((b.y - a.y) / mm), (float)((b.z - a.z) / mm),
-                        (float)((b.w - a.w) / mm), (float)mm);
+        cxd = (b.v[1] - a.v[1]) / mm;
+        cyd = (b.v[2] - a.v[2]) / mm;
+        czd = (b.v[3] - a.v[3]) / mm;
+        c = make_float4((float)cxd, (float)cyd, (float)czd, (float)mm);
     } else {
         const float4 p = bodies[T.off[s] + lo], q = bodies[T.off[s] + hi];
         c = make_float4(0.5f * (p.x + q.x), 0.5f * (p.y + q.y), 0.5f * (p.z + q.z), 0.f);
     }
     bin_com[j] = c;
+    if (P2) {
+        // traceless quadrupole about the COM: Q = 3 C - tr(C) I with the
+        // central second moment C = S - M c c^T of the range sums S
+        const DVec<6> u = P2[T.off[s] + lo], o = P2[T.off[s] + hi + 1];
+        const double sxx = o.v[0] - u.v[0] - mm * cxd * cxd;
+        const double syy = o.v[1] - u.v[1] - mm * cyd * cyd;
+        const double szz = o.v[2] - u.v[2] - mm * czd * czd;
+        const double sxy = o.v[3] - u.v[3] - mm * cxd * cyd;
+        const double sxz = o.v[4] - u.v[4] - mm * cxd * czd;
+        const double syz = o.v[5] - u.v[5] - mm * cyd * czd;
+        const double tr = sxx + syy + szz;
+        bin_q[2 * j] = make_float4((float)(3 * sxx - tr), (float)(3 * syy - tr),
+                                   (float)(3 * szz - tr), (float)(3 * sxy));
+        bin_q[2 * j + 1] = make_float4((float)(3 * sxz), (float)(3 * syz), 0.f, 0.f);
+    }
     // cell centre: descend `lev` octant digits of this node's prefix
     const Root R = roots[s];
     float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
@@ -679,9 +923,20 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     SegTable T, const int *__restrict__ left, const int *__restrict__ right,
     const int *__restrict__ octf, const int *__restrict__ cnt, const int *__restrict__ first,
     const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
-    const float4 *__restrict__ bodies, float4 *rec_a, int4 *rec_b, float4 *rec_c)
+    const float4 *__restrict__ bodies, const float4 *__restrict__ bin_q, float4 *rec_a,
+    int4 *rec_b, float4 *rec_c, float4 *rec_q)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    auto put_q = [&](int code, int r) {
+        if (!rec_q) return;
+        if (code < 0) {
+            rec_q[2 * r] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rec_q[2 * r + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            rec_q[2 * r] = bin_q[2 * code];
+            rec_q[2 * r + 1] = bin_q[2 * code + 1];
+        }
+    };
     auto rec_of = [&](int code, float4 &a, int4 &b, float4 &c) {
         if (code < 0) {
             const int p = ~code;
@@ -701,6 +956,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_a[T.noff[j]] = a;
         rec_b[T.noff[j]] = b;
         rec_c[T.noff[j]] = c;
+        put_q(~T.off[j], T.noff[j]);
     }
     if (j >= T.ioff[T.nseg]) return;
     if (cnt[j] == 0) return;
@@ -712,6 +968,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_a[T.noff[s]] = a;
         rec_b[T.noff[s]] = b;
         rec_c[T.noff[s]] = c;
+        put_q(j, T.noff[s]);
     }
     const int f0 = first[j];
     oct_children(left, right, octf, j, [&](int k, int code) {
@@ -721,6 +978,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_a[f0 + k] = a;
         rec_b[f0 + k] = b;
         rec_c[f0 + k] = c;
+        put_q(code, f0 + k);
     });
 }
 
@@ -777,10 +1035,38 @@ void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, flo
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
                       const float *az, const float *dv, const float *x, const float *z,
                       float dt_half, int has_drag, float decay, float star_mass, int n,
-                      hipStream_t st)
+                      int *step_ctr, hipStream_t st)
 {
     hipLaunchKernelGGL(k_kick_drag, NBX_GRID(n), vx, vy, vz, ax, ay, az, dv, x, z, dt_half,
-                       has_drag, decay, star_mass, n);
+                       has_drag, decay, star_mass, n, step_ctr);
+}
+
+void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,
+                              float *vy, float *vz, const float *ax, const float *ay,
+                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kick_drift_bbox_c, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.lo, B.partial, B.roots, B.tickets);
+}
+
+void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const float *ay,
+                        const float *az, const float *dv, const float *x, const float *z,
+                        float dt_half, int has_drag, double decay, float star_mass, int n,
+                        int *step_ctr, Bufs &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kick_drag_c, NBX_GRID(n), vx, vy, vz, ax, ay, az, dv, x, z, dt_half,
+                       has_drag, decay, star_mass, n, step_ctr, B.lo);
+}
+
+void launch_diagnostics(const float *x, const float *y, const float *z, const float *vx,
+                        const float *vy, const float *vz, const float *m, const float *lo, int n,
+                        double star_mass, double soft, int with_self, Bufs &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, B.diag, 0.0, 8);
+    hipLaunchKernelGGL(k_diag, NBX_GRID(n), x, y, z, vx, vy, vz, m, lo, n, star_mass, soft,
+                       B.diag);
+    if (with_self)
+        hipLaunchKernelGGL(k_diag_self, NBX_GRID(n), x, y, z, m, lo, n, soft, B.diag);
 }
 
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
@@ -799,7 +1085,7 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
     const int nst = B.n_sweep_tiles * 256;
     hipLaunchKernelGGL(k_keys, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.roots,
                        B.keys[0], B.vals[0], B.ghist, B.status[0], nst, B.pflag,
-                       B.n_scan_tiles, B.tickets);
+                       B.n_scan_tiles, B.pflag2, B.tickets);
     const int npass = kPasses + (T.nseg > 1 ? 1 : 0);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
@@ -811,17 +1097,18 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
                            B.tickets, last ? 1 : 0, x, y, z, mass, B.bodies);
         cur ^= 1;
     }
-    if (cur != 0) {  // keep the sorted arrays in buffer 0
-        (void)hipMemcpyAsync(B.keys[0], B.keys[1], sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
-                             st);
-        (void)hipMemcpyAsync(B.vals[0], B.vals[1], sizeof(int) * n, hipMemcpyDeviceToDevice, st);
-    }
-    hipLaunchKernelGGL(k_prefix, dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies, n,
-                       B.prefix, B.pflag, B.pagg, B.pinc, B.tickets, B.ghist);
+    B.cur = cur;
+    hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies, n,
+                       B.prefix, B.pflag, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist);
+    if (B.quad)
+        hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies,
+                           n, B.prefix2, B.pflag2, B.pagg2, B.pinc2, B.tickets, kTicketScan2,
+                           B.ghist);
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
-        hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[0], B.prefix, B.bodies, B.roots,
-                           B.left, B.right, B.plen, B.oct, B.bin_com, B.bin_cell);
+        hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[B.cur], B.prefix,
+                           B.quad ? B.prefix2 : nullptr, B.bodies, B.roots, B.left, B.right,
+                           B.plen, B.oct, B.bin_com, B.bin_cell, B.bin_q);
 }
 
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
@@ -832,8 +1119,8 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
                            B.first, B.alloc, B.maxlev, B.err);
     hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.left, B.right,
-                       B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.rec_a, B.rec_b,
-                       B.rec_c);
+                       B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a,
+                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr);
 }
 
 }  // namespace nbx

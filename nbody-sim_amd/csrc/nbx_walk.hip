@@ -13,6 +13,8 @@
 // Collision and FoF (per lane): the search radius is far below the particle
 // spacing, so the per-body paths diverge immediately; each lane descends on
 // its own stack, pruning cells by the exact distance to the cell box.
+#include <type_traits>
+
 #include "nbx_device.hpp"
 #include "nbx_launch.hpp"
 
@@ -37,11 +39,23 @@ struct WaveLds {
     unsigned long long m[kWave];
 };
 
-// Visitor: visit(a, b, leaf) for one record on the lanes that own it;
+struct WaveLdsQ : WaveLds {
+    float4 q[2 * kWave];
+};
+
+// Record data handed to a visitor (quadrupole rows only with QUAD).
+struct Rec {
+    float4 a;
+    int4 b;
+    float4 q0, q1;
+};
+
+// Visitor: visit(rec, leaf) for one record on the lanes that own it;
 // returns true when the lane opens the (internal) record.
-template <class Visitor>
+template <bool QUAD, class Lds, class Visitor>
 __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
-                                 int root, uint64_t active, WaveLds &L, int *err, Visitor &vis)
+                                 const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
+                                 int *err, Visitor &vis)
 {
     const int lane = __lane_id();
     // every lane writes the same entry: per-lane program order covers the
@@ -72,16 +86,26 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
             L.a[lane] = ra[my_rec];
             L.b[lane] = rb[my_rec];
             L.m[lane] = my_mask;
+            if constexpr (QUAD) {
+                L.q[2 * lane] = rq[2 * my_rec];
+                L.q[2 * lane + 1] = rq[2 * my_rec + 1];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         for (int k = 0; k < nrec; ++k) {
-            const float4 a = L.a[k];
-            const int4 b = L.b[k];
+            Rec r;
+            r.a = L.a[k];
+            r.b = L.b[k];
+            if constexpr (QUAD) {
+                r.q0 = L.q[2 * k];
+                r.q1 = L.q[2 * k + 1];
+            }
+            const int4 b = r.b;
             const uint64_t mk = L.m[k];
             const bool leaf = b.w == 0;
-            const bool open = ((mk >> lane) & 1ull) && vis.visit(a, b, leaf);
+            const bool open = ((mk >> lane) & 1ull) && vis.visit(r, leaf);
             const uint64_t om = ballot64(open);
             if (om) {
                 if (sp < kGStack) {
@@ -103,21 +127,38 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
 // half-width / COM-distance test of BHTree.cu:576-577); skip the target's
 // own leaf (BHTree.cu:580); monopole pull m d / (r^2+eps^2)^{3/2}.
 // ---------------------------------------------------------------------------
+template <bool QUAD>
 struct GravityVisitor {
     float bx, by, bz, theta2, eps2;
     int self;
     float fx, fy, fz;
-    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
+    __device__ inline bool visit(const Rec &r, bool leaf)
     {
+        const float4 &a = r.a;
+        const int4 &b = r.b;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
         if (!open && !(leaf && b.z == self)) {
             const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
-            const float f = a.w * rinv * rinv * rinv;
+            const float rinv2 = rinv * rinv;
+            const float rinv3 = rinv2 * rinv;
+            const float f = a.w * rinv3;
             fx += f * dx;
             fy += f * dy;
             fz += f * dz;
+            if constexpr (QUAD) {
+                // a += -Q R / r^5 + (5/2) (R.Q R) R / r^7 for R = COM - body
+                // (Q traceless; zero for leaves)
+                const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
+                const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
+                const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
+                const float rinv5 = rinv3 * rinv2;
+                const float g = 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2;
+                fx += g * dx - rinv5 * qx;
+                fy += g * dy - rinv5 * qy;
+                fz += g * dz - rinv5 * qz;
+            }
         }
         return open;
     }
@@ -129,8 +170,10 @@ struct CountVisitor {
     float bx, by, bz, theta2;
     int self;
     int n_acc, n_open;
-    __device__ inline bool visit(const float4 &a, const int4 &b, bool leaf)
+    __device__ inline bool visit(const Rec &r, bool leaf)
     {
+        const float4 &a = r.a;
+        const int4 &b = r.b;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
@@ -140,13 +183,16 @@ struct CountVisitor {
     }
 };
 
+template <bool QUAD>
 __global__ __launch_bounds__(kWalkBlock) void k_walk(
-    SegTable T, DiskTable D, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
+    SegTable T, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
+    const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
     float *saz, int *err)
 {
-    __shared__ WaveLds lds[kWalkWaves];
+    using Lds = typename std::conditional<QUAD, WaveLdsQ, WaveLds>::type;
+    __shared__ Lds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;  // uniform per wave
@@ -154,8 +200,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
-    GravityVisitor v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), lds[wib], err, v);
+    GravityVisitor<QUAD> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
+    wave_walk<QUAD>(ra, rb, rq, T.noff[s], ballot64(valid), lds[wib], err, v);
     if (!valid) return;
     const int i = perm[pos];
     float ox, oy, oz;
@@ -169,7 +215,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
         ox = f * dx + v.fx;
         oy = f * dy + v.fy;
         oz = f * dz + v.fz;
-        const DiskDev &d = D.d[s];
+        DiskDev d{};
+        if (dtab) d = dtab[*step_ctr * kMaxSeg + s];
         if (d.pmass > 0.f)
             perturber_add_dev(b.x, b.y, b.z, d.px, d.py, d.pz, d.pmass, softening, ox, oy, oz);
     }
@@ -199,7 +246,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(SegTable T, const fl
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
     CountVisitor v{b.x, b.y, b.z, theta2, pos, 0, 0};
-    wave_walk(ra, rb, T.noff[s], ballot64(valid), lds[wib], err, v);
+    wave_walk<false>(ra, rb, nullptr, T.noff[s], ballot64(valid), lds[wib], err, v);
     if (!valid) return;
     const int i = perm[pos];
     n_acc[i] = v.n_acc;
@@ -430,12 +477,13 @@ __global__ void k_fof_final(int *parent, int *labels, int n)
 // Launchers
 // ===========================================================================
 
-void launch_walk(const SegTable &T, const DiskTable &D, const WalkArgs &W, Bufs &B,
-                 hipStream_t st)
+void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
+                 Bufs &B, hipStream_t st)
 {
     const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_walk, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
-                       B.rec_a, B.rec_b, B.bodies, B.vals[0], W.theta2, W.eps2, W.softening,
+    auto kern = W.quad ? k_walk<true> : k_walk<false>;
+    hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, dtab,
+                       step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies, B.vals[B.cur], W.theta2, W.eps2, W.softening,
                        W.star_mass, W.ax, W.ay, W.az, W.sax, W.say, W.saz, B.err);
 }
 
@@ -444,7 +492,7 @@ void launch_walk_counts(const SegTable &T, float theta2, int *n_acc, int *n_open
 {
     const int waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_walk_counts, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
-                       B.rec_a, B.rec_b, B.bodies, B.vals[0], theta2, n_acc, n_open, B.err);
+                       B.rec_a, B.rec_b, B.bodies, B.vals[B.cur], theta2, n_acc, n_open, B.err);
 }
 
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,
@@ -452,7 +500,7 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
 {
     const int waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
-                       B.rec_c, B.rec_b, B.bodies, B.vals[0], vx, vy, vz, B.roots, restitution,
+                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], vx, vy, vz, B.roots, restitution,
                        B.dv, B.err);
 }
 
@@ -461,7 +509,7 @@ void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t s
     const int n = T.n_total, waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels, n);
     hipLaunchKernelGGL(k_fof_link, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
-                       B.rec_c, B.rec_b, B.bodies, B.vals[0], B.roots, ll, B.labels, B.err);
+                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], B.roots, ll, B.labels, B.err);
     hipLaunchKernelGGL(k_fof_final, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels,
                        labels, n);
 }

@@ -78,6 +78,7 @@ EXPORTS = [
     "nbx_force_ex", "nbx_collide", "nbx_perturber_force", "nbx_half_kick", "nbx_drift",
     "nbx_reset_acceleration", "nbx_gas_drag", "nbx_fof", "nbx_init_forces", "nbx_step",
     "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_walk_counts",
+    "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
 ]
 
 _lib = None
@@ -112,6 +113,9 @@ def lib():
             "nbx_tree_stats": [vp, vp, vp], "nbx_tree_set_timing": [vp, i32],
             "nbx_phase_ms": [vp, C.c_char_p, C.POINTER(f32)],
             "nbx_walk_counts": [vp, f32, vp, vp],
+            "nbx_tree_set_multipole": [vp, i32],
+            "nbx_tree_set_precision": [vp, i32],
+            "nbx_diagnostics": [vp, P, f32, f32, i32, C.POINTER(C.c_double)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -483,6 +487,26 @@ def init_forces(t, p, sp, dps):
 def step(t, p, sp, dps, step0, n_steps):
     _check(lib().nbx_step(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps), step0, n_steps),
            "step")
+
+
+def set_multipole(t, order):
+    """1 = monopole (reference), 2 = monopole + quadrupole of accepted nodes."""
+    _check(lib().nbx_tree_set_multipole(t.h, int(order)), "set_multipole")
+
+
+def set_precision(t, compensated):
+    """True: FP64-compensated KDK updates (forces stay FP32)."""
+    _check(lib().nbx_tree_set_precision(t.h, int(bool(compensated))), "set_precision")
+
+
+def diagnostics(t, p, star_mass=1.0, softening=1e-3, with_self=False):
+    """FP64 energy and angular-momentum sums of the state."""
+    out = (C.c_double * 7)()
+    _check(lib().nbx_diagnostics(t.h, C.byref(p.c()), star_mass, softening, int(with_self), out),
+           "diagnostics")
+    return dict(kinetic=out[0], star_potential=out[1], L=(out[2], out[3], out[4]),
+                self_potential=out[5], mass=out[6],
+                energy=out[0] + out[1] + out[5])
 
 
 def set_async(t, on):

@@ -378,3 +378,123 @@ def test_argument_errors_raise(gpu):
     t2 = nbx.allocateOctree(100)
     with pytest.raises(RuntimeError, match="before"):
         nbx.launchForceKernel(t2, p, 0.5, 1e-3, 1.0)
+
+
+def test_graph_replay_equals_direct_launches(gpu):
+    """The captured step replayed with the device step counter gives the same
+    state bit for bit as direct launches, across several replays with eager
+    work (a FoF pass and a synchronisation) between them."""
+    import torch
+    nbx = gpu
+    d = disk(5000, mass=1e-5, seed=31)
+    sp_d, dp, r_coll = _c1_params(nbx, perturber=True)
+    sp_g = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                           gas_drag_rate=1e-3, collisions=True, use_graph=True)
+    pa, pb = nbx.to_gpu(d), nbx.to_gpu(d)
+    ta, tb = nbx.allocateOctree(pa.n), nbx.allocateOctree(pb.n)
+    nbx.init_forces(ta, pa, sp_d, dp)
+    nbx.init_forces(tb, pb, sp_g, dp)
+    lab = torch.empty(pb.n, dtype=torch.int32, device="cuda")
+    step = 0
+    for k in (3, 1, 4):
+        nbx.step(ta, pa, sp_d, dp, step, k)
+        nbx.step(tb, pb, sp_g, dp, step, k)
+        nbx.fof_into(tb, pb, r_coll, lab)
+        torch.cuda.synchronize()
+        step += k
+        ga, gb = pa.numpy(), pb.numpy()
+        for key in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            np.testing.assert_array_equal(ga[key], gb[key], err_msg=f"{key} after step {step}")
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.25])
+def test_quadrupole_force_vs_oracle(gpu, theta):
+    """Multipole order 2: same accepted set, each accepted node adds its
+    traceless quadrupole.  Against the oracle's quadrupole walk: median rel
+    diff <= 1e-5, 99.9th pct <= 1e-3; against FP64 direct sum the median
+    error drops below the monopole's (at theta 0.25 below 1e-3)."""
+    import torch
+    nbx = gpu
+    n = 32768
+    d = disk(n, mass=3e-5, seed=41)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(n)
+    nbx.set_multipole(t, 2)
+    t = _build(nbx, p, t)
+    sa = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    nbx.launchForceKernel(t, p, theta, 1e-3, 1.0, self_acc=sa)
+    g = np.stack([_np(s) for s in sa], 1)
+    mt = O.mt_force(d["x"], d["y"], d["z"], d["mass"], theta, 1e-3, quad=True)
+    e = rel_err(g, mt["self_a"])
+    assert np.median(e) <= 1e-5, np.median(e)
+    assert np.percentile(e, 99.9) <= 1e-3, np.percentile(e, 99.9)
+    tg = np.sort(np.random.default_rng(1).choice(n, 512, replace=False)).astype(np.int32)
+    ad = O.direct_f64(d["x"], d["y"], d["z"], d["mass"], 1e-3, targets=tg)
+    mono = O.mt_force(d["x"], d["y"], d["z"], d["mass"], theta, 1e-3)["self_a"]
+    eq, em = np.median(rel_err(g[tg], ad)), np.median(rel_err(mono[tg], ad))
+    assert eq < 0.6 * em, (eq, em)
+    if theta <= 0.25:
+        assert eq <= 1e-3, eq
+
+
+def _kepler_fp64(x, v, dt, steps, soft=1e-3):
+    """All-FP64 KDK of one body around the unit star (same force law)."""
+    x = np.array(x, np.float64)
+    v = np.array(v, np.float64)
+
+    def acc(x):
+        r2 = x @ x + soft * soft
+        return -x / (r2 * np.sqrt(r2))
+
+    a = acc(x)
+    for _ in range(steps):
+        v = v + a * (dt / 2)
+        x = x + v * dt
+        a = acc(x)
+        v = v + a * (dt / 2)
+    return x, v
+
+
+def test_compensated_kdk_tracks_fp64(gpu):
+    """Precision mode: the compensated FP64 KDK stays closer to an all-FP64
+    integration than the plain float KDK (one body, 3000 steps, e = 0.3)."""
+    nbx = gpu
+    f = np.float32
+    d = {k: np.zeros(1, f) for k in nbx.FIELDS}
+    d["x"][0], d["vz"][0], d["mass"][0] = 1.0, 1.14, 1e-9
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, collisions=False)
+    dp = nbx.disk_params(0.0)
+    xr, vr = _kepler_fp64([1.0, 0.0, 0.0], [0.0, 0.0, float(f(1.14))], 0.01, 3000)
+    errs = {}
+    for comp in (False, True):
+        p = nbx.to_gpu(d)
+        t = nbx.allocateOctree(1)
+        nbx.set_precision(t, comp)
+        nbx.init_forces(t, p, sp, dp)
+        nbx.step(t, p, sp, dp, 0, 3000)
+        g = p.numpy()
+        errs[comp] = np.linalg.norm(np.array([g["x"][0], g["y"][0], g["z"][0]], np.float64) - xr)
+    assert errs[True] < 0.5 * errs[False], errs
+
+
+def test_fp64_diagnostics(gpu):
+    """Energy / angular momentum sums against numpy float64."""
+    nbx = gpu
+    d = disk(3000, mass=1e-5, seed=51)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(p.n)
+    out = nbx.diagnostics(t, p, 1.0, 1e-3, with_self=True)
+    X = np.stack([d["x"], d["y"], d["z"]], 1).astype(np.float64)
+    V = np.stack([d["vx"], d["vy"], d["vz"]], 1).astype(np.float64)
+    m = d["mass"].astype(np.float64)
+    ke = 0.5 * (m * (V * V).sum(1)).sum()
+    pe = -(m / np.sqrt((X * X).sum(1) + 1e-6)).sum()
+    L = (m[:, None] * np.cross(X, V)).sum(0)
+    D = X[:, None, :] - X[None, :, :]
+    R = np.sqrt((D * D).sum(-1) + 1e-6)
+    np.fill_diagonal(R, np.inf)
+    ps = -0.5 * (m[:, None] * m[None, :] / R).sum()
+    assert abs(out["kinetic"] - ke) <= 1e-12 * abs(ke)
+    assert abs(out["star_potential"] - pe) <= 1e-12 * abs(pe)
+    assert np.allclose(out["L"], L, rtol=1e-12, atol=1e-20)
+    assert abs(out["self_potential"] - ps) <= 1e-10 * abs(ps)
