@@ -97,6 +97,7 @@ def main():
     t = nbx.allocateOctree(n)
     labels = torch.empty(n, dtype=torch.int32, device="cuda")
     nbx.init_forces(t, p, sp, dp)
+    tune_ms = nbx.tune_walk(t, 0.5, 1e-3, 1.0)
     nbx.set_async(t, True)
 
     def run(step0, k):
@@ -158,6 +159,8 @@ def main():
         "roofline": dominant,
         "roofline_walk": roof_walk,
         "phases_ms": phases,
+        "walk_shape": {"targets_per_wave": nbx.walk_shape(t),
+                       "tuned_ms": {str(k): v for k, v in tune_ms.items()}},
         "force_err": acc_stats,
     }
     if not args.no_batch:
@@ -190,6 +193,7 @@ def batched_sweep(nbx, np, torch, make_disk, args):
         p = nbx.to_gpu(dd)
         t = nbx.allocateOctree(p.n)
         nbx.init_forces(t, p, sp, dp)
+        nbx.tune_walk(t, 0.5, 1e-3, 1.0)
         nbx.set_async(t, True)
         plain.append((p, t, dp))
     for p, t, dp in plain:  # warm-up (graph capture)
@@ -207,6 +211,7 @@ def batched_sweep(nbx, np, torch, make_disk, args):
     pb = nbx.to_gpu(cat)
     tb = nbx.allocateOctreeBatch(SWEEP_N)
     nbx.init_forces(tb, pb, sp, dps)
+    nbx.tune_walk(tb, 0.5, 1e-3, 1.0)
     nbx.set_async(tb, True)
     nbx.step(tb, pb, sp, dps, 0, 5)
     nbx.sync(tb)
@@ -218,6 +223,8 @@ def batched_sweep(nbx, np, torch, make_disk, args):
     bat_s = time.perf_counter() - t0
     total = float(sum(SWEEP_N))
     return {"disks": SWEEP_N, "perturber_mjup": SWEEP_MJUP, "steps": k,
+            "walk_shape_batched": nbx.walk_shape(tb),
+            "walk_shape_plain": [nbx.walk_shape(t) for _, t, _ in plain],
             "batched_particle_steps_per_s": total * k / bat_s,
             "sequential_particle_steps_per_s": total * k / seq_s,
             "speedup_vs_sequential": seq_s / bat_s,

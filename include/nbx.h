@@ -116,6 +116,10 @@ int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, fl
  * (self excluded) and opened internal records of the theta walk on the last
  * summarized tree, written to device arrays in original body order. */
 int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n_open);
+/* No reference counterpart (diagnostic): statistics of the theta walk with the
+ * tree's current shape, host out[4] = {mean accepted per body, mean opened
+ * per body, mean records evaluated per wavefront, deepest wave stack}. */
+int nbx_walk_stats(nbx_tree *t, float theta, double *out);
 /* launchCollisionKernel (BHTree.cu:688-708): exact pair set
  * dist^2 < r_coll^2, impulse J = -(1+e) dvn / 2 for approaching pairs,
  * deltas accumulated per body in a fixed order, then v += dv. */
@@ -160,6 +164,20 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
  * updatePerturber (Perturber.h:60-66) evaluated on the host. */
 int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
              const nbx_disk_params *dp, int32_t step0, int32_t n_steps);
+
+/* ---- walk shape and autotuner (no reference counterpart) ----------------- */
+/* Digit width of the radix passes of the key sort: 8, 9 or 11 bits (8, 7 or
+ * 6 passes over the 63-bit keys).  The sorted order does not depend on it. */
+int nbx_tree_set_radix_bits(nbx_tree *t, int bits);
+/* Targets per wavefront of the gravity walk (64, 32, 16 or 8; the 64 lanes
+ * of a wave split each batch of tree records over 64/targets slots).  Results
+ * depend on the shape only through the summation order. */
+int nbx_tree_set_walk_shape(nbx_tree *t, int targets_per_wave);
+int nbx_tree_get_walk_shape(nbx_tree *t, int32_t *targets_per_wave);
+/* Times the gravity walk of every shape on the last summarized tree (outputs
+ * go to scratch, the state is not touched), keeps the fastest, and writes
+ * the mean ms per walk of shapes 64, 32, 16, 8 to ms_out[4] (may be NULL). */
+int nbx_tune_walk(nbx_tree *t, float theta, float softening, float star_mass, float *ms_out);
 
 /* ---- precision (no reference counterpart) -------------------------------- */
 /* compensated=1: the KDK position / velocity updates (and the drag) of

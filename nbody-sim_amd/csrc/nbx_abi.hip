@@ -41,6 +41,9 @@ int fail(int code, const char *fmt, ...)
 
 enum Stage { kNone = 0, kBBox = 1, kBuilt = 2, kSummarized = 3 };
 
+constexpr int kDefaultTargets = 16;
+const int kShapes[4] = {64, 32, 16, 8};
+
 }  // namespace
 
 struct nbx_tree {
@@ -62,6 +65,8 @@ struct nbx_tree {
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_key;
     bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
+    int walk_tg = kDefaultTargets;  // gravity-walk targets per wavefront
+    float tune_ms[4] = {0, 0, 0, 0};  // autotuner timings for 64, 32, 16, 8 targets
 };
 
 namespace {
@@ -114,7 +119,7 @@ int alloc_bufs(nbx_tree *t)
     const int n = t->T.n_total;
     B.n = n;
     B.n_sweep_tiles = (n + kSweepTile - 1) / kSweepTile;
-    B.n_scan_tiles = (n + kScanTile - 1) / kScanTile;
+    B.n_scan_tiles = t->T.stile[t->T.nseg];
     int rc = NBX_OK;
 #define ALLOC(ptr, cnt) \
     if ((rc = dmalloc(&(ptr), (size_t)(cnt))) != NBX_OK) return rc
@@ -122,11 +127,11 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.keys[1], n);
     ALLOC(B.vals[0], n);
     ALLOC(B.vals[1], n);
-    ALLOC(B.ghist, kPasses * 256);
-    ALLOC(B.status[0], 256 * B.n_sweep_tiles);
-    ALLOC(B.status[1], 256 * B.n_sweep_tiles);
+    ALLOC(B.ghist, kHistInts);
+    ALLOC(B.status[0], 2048 * B.n_sweep_tiles);
+    ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
     ALLOC(B.tickets, 32);
-    ALLOC(B.prefix, n + 1);
+    ALLOC(B.prefix, n + t->T.nseg);
     ALLOC(B.pflag, B.n_scan_tiles);
     ALLOC(B.pagg, B.n_scan_tiles);
     ALLOC(B.pinc, B.n_scan_tiles);
@@ -158,7 +163,7 @@ int alloc_bufs(nbx_tree *t)
     HIPCHK(hipMemset(B.rec_c, 0, sizeof(float4) * B.n_rec));
     HIPCHK(hipMemset(B.err, 0, 4 * sizeof(int)));
     HIPCHK(hipMemset(B.dv, 0, 3 * sizeof(float) * n));
-    HIPCHK(hipMemset(B.ghist, 0, kPasses * 256 * sizeof(int)));
+    HIPCHK(hipMemset(B.ghist, 0, kHistInts * sizeof(int)));
     HIPCHK(hipMemset(B.tickets, 0, 32 * sizeof(int)));
     HIPCHK(hipMemset(B.alloc, 0, 2 * kMaxSeg * sizeof(int)));
     HIPCHK(hipHostMalloc((void **)&t->h_err, 4 * sizeof(int)));
@@ -221,9 +226,10 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
     }
     T.off[nseg] = (int)total;
     T.n_total = (int)total;
-    T.blk[0] = T.wave[0] = T.ioff[0] = T.noff[0] = 0;
+    T.blk[0] = T.wave[0] = T.ioff[0] = T.noff[0] = T.stile[0] = 0;
     for (int s = 0; s < nseg; ++s) {
         const int n = counts[s];
+        T.stile[s + 1] = T.stile[s] + (n + kScanTile - 1) / kScanTile;
         T.blk[s + 1] = T.blk[s] + (n + kBlock - 1) / kBlock;
         T.wave[s + 1] = T.wave[s] + (n + kWave - 1) / kWave;
         T.ioff[s + 1] = T.ioff[s] + (n - 1);
@@ -235,6 +241,7 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
         T.wave[s] = T.wave[nseg];
         T.ioff[s] = T.ioff[nseg];
         T.noff[s] = T.noff[nseg];
+        T.stile[s] = T.stile[nseg];
     }
     hipError_t e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
@@ -427,7 +434,7 @@ int nbx_tree_set_multipole(nbx_tree *t, int order)
     if (order == 2 && !B.prefix2) {
         const int n = t->T.n_total;
         int rc;
-        if ((rc = dmalloc(&B.prefix2, (size_t)n + 1)) != NBX_OK) return rc;
+        if ((rc = dmalloc(&B.prefix2, (size_t)n + t->T.nseg)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.pagg2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.pinc2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.bin_q, (size_t)2 * n)) != NBX_OK) return rc;
@@ -481,6 +488,70 @@ int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float 
                        t->B, t->stream);
     HIPCHK(hipMemcpyAsync(out, t->B.diag, 7 * sizeof(double), hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
+    return finish(t);
+}
+
+int nbx_tree_set_radix_bits(nbx_tree *t, int bits)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (bits != 8 && bits != 9 && bits != 11) return fail(NBX_E_ARG, "radix bits %d not in {8, 9, 11}", bits);
+    t->B.radix_bits = bits;
+    t->stage = kNone;
+    if (t->graph) {
+        HIPCHK(hipGraphExecDestroy(t->graph));
+        t->graph = nullptr;
+    }
+    return NBX_OK;
+}
+
+int nbx_tree_set_walk_shape(nbx_tree *t, int targets_per_wave)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    bool ok = false;
+    for (int k : kShapes) ok |= k == targets_per_wave;
+    if (!ok) return fail(NBX_E_ARG, "targets per wave %d not in {64, 32, 16, 8}", targets_per_wave);
+    t->walk_tg = targets_per_wave;
+    return NBX_OK;
+}
+
+int nbx_tree_get_walk_shape(nbx_tree *t, int32_t *targets_per_wave)
+{
+    if (!t || !targets_per_wave) return fail(NBX_E_ARG, "null argument");
+    *targets_per_wave = t->walk_tg;
+    return NBX_OK;
+}
+
+int nbx_tune_walk(nbx_tree *t, float theta, float softening, float star_mass, float *ms_out)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk tuning before summarize");
+    const int n = t->T.n_total;
+    float *sc = t->B.dv;  // scratch outputs: the tuner never touches the state
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = t->walk_tg;
+    float best_ms = 1e30f;
+    for (int q = 0; q < 4; ++q) {
+        WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad,
+                   kShapes[q], sc, sc + n, sc + 2 * n, nullptr, nullptr, nullptr};
+        launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);  // warm-up
+        HIPCHK(hipEventRecord(e0, t->stream));
+        for (int rep = 0; rep < 3; ++rep) launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);
+        HIPCHK(hipEventRecord(e1, t->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        t->tune_ms[q] = ms / 3.f;
+        if (ms_out) ms_out[q] = ms / 3.f;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = kShapes[q];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    t->walk_tg = best;
     return finish(t);
 }
 
@@ -563,7 +634,8 @@ int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, fl
     if (rc) return rc;
     if (t->stage < kSummarized) return fail(NBX_E_STATE, "launchForceKernel before summarize");
     if (!(theta > 0.f)) return fail(NBX_E_ARG, "theta must be positive");
-    WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad, p->ax, p->ay,
+    WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad, t->walk_tg,
+               p->ax, p->ay,
                p->az, self_ax, self_ay, self_az};
     launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);
     return finish(t);
@@ -578,7 +650,44 @@ int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n
 {
     if (!t || !dev_n_acc || !dev_n_open) return fail(NBX_E_ARG, "null argument");
     if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk counts before summarize");
-    launch_walk_counts(t->T, theta * theta, dev_n_acc, dev_n_open, t->B, t->stream);
+    launch_walk_counts(t->T, theta * theta, 64, dev_n_acc, dev_n_open, nullptr, t->B, t->stream);
+    return finish(t);
+}
+
+int nbx_walk_stats(nbx_tree *t, float theta, double *out)
+{
+    if (!t || !out) return fail(NBX_E_ARG, "null argument");
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk statistics before summarize");
+    const int n = t->T.n_total;
+    int *acc = nullptr;
+    unsigned long long *pop = nullptr;
+    int rc = dmalloc(&acc, (size_t)2 * n);
+    if (rc) return rc;
+    if ((rc = dmalloc(&pop, 1)) != NBX_OK) {
+        (void)hipFree(acc);
+        return rc;
+    }
+    HIPCHK(hipMemsetAsync(pop, 0, sizeof(*pop), t->stream));
+    HIPCHK(hipMemsetAsync(t->B.err + 2, 0, sizeof(int), t->stream));
+    launch_walk_counts(t->T, theta * theta, t->walk_tg, acc, acc + n, pop, t->B, t->stream);
+    std::vector<int> h(2 * n);
+    unsigned long long hp = 0;
+    int depth = 0;
+    HIPCHK(hipMemcpyAsync(h.data(), acc, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipMemcpyAsync(&hp, pop, sizeof(hp), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipMemcpyAsync(&depth, t->B.err + 2, sizeof(int), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    (void)hipFree(acc);
+    (void)hipFree(pop);
+    double sa = 0, so = 0;
+    for (int i = 0; i < n; ++i) {
+        sa += h[i];
+        so += h[n + i];
+    }
+    out[0] = sa / n;
+    out[1] = so / n;
+    out[2] = (double)hp / walk_waves(t->T, t->walk_tg);
+    out[3] = depth;
     return finish(t);
 }
 
@@ -662,7 +771,7 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
     WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, t->B.quad, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
     rc = upload_steps(t, dp, 0, 1, sp->dt, true);
     if (rc) return rc;
     launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
@@ -676,7 +785,7 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     const int has_drag = sp->gas_drag_rate > 0.0f;
     const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
     WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, t->B.quad, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
     if (e0 >= 0) ev_record(t, e0 + 0);
     if (t->compensated)
         launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
@@ -707,9 +816,9 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
 
 // Everything a captured step bakes into its kernel arguments.
 static std::vector<unsigned char> step_key(const nbx_particles *p, const nbx_step_params *sp,
-                                           const DiskTable &D, const Bufs &B)
+                                           const DiskTable &D, const Bufs &B, int targets)
 {
-    std::vector<unsigned char> k(sizeof(*p) + sizeof(*sp) + sizeof(D) + sizeof(void *) + 1);
+    std::vector<unsigned char> k(sizeof(*p) + sizeof(*sp) + sizeof(D) + sizeof(void *) + 2);
     unsigned char *q = k.data();
     memcpy(q, p, sizeof(*p));
     q += sizeof(*p);
@@ -719,7 +828,8 @@ static std::vector<unsigned char> step_key(const nbx_particles *p, const nbx_ste
     q += sizeof(D);
     memcpy(q, &B.dtab, sizeof(void *));
     q += sizeof(void *);
-    *q = B.lo != nullptr;
+    q[0] = B.lo != nullptr;
+    q[1] = (unsigned char)targets;
     return k;
 }
 
@@ -736,7 +846,7 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     const DiskTable D = disk_rcoll(t, dp);
     const bool graph = sp->use_graph && !t->timing;
     if (graph) {
-        std::vector<unsigned char> key = step_key(p, sp, D, t->B);
+        std::vector<unsigned char> key = step_key(p, sp, D, t->B, t->walk_tg);
         if (!t->graph || key != t->graph_key) {
             if (t->graph) {
                 HIPCHK(hipGraphExecDestroy(t->graph));

@@ -34,6 +34,7 @@ struct SegTable {
     int wave[kMaxSeg + 1];       // wave (64 targets) offset of disk s
     int ioff[kMaxSeg + 1];       // binary internal-node offset (n_s - 1 each)
     int noff[kMaxSeg + 1];       // walk-record offset (2 n_s + 1 each)
+    int stile[kMaxSeg + 1];      // prefix-scan tile offset (tiles never straddle disks)
 };
 
 __host__ __device__ inline int seg_of_block(const SegTable &T, int b)
@@ -52,6 +53,12 @@ __host__ __device__ inline int seg_of_wave(const SegTable &T, int w)
 {
     int s = 0;
     while (s + 1 < T.nseg && w >= T.wave[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_stile(const SegTable &T, int t)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && t >= T.stile[s + 1]) ++s;
     return s;
 }
 __host__ __device__ inline int seg_of_internal(const SegTable &T, int j)

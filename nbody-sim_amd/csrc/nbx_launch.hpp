@@ -19,8 +19,9 @@ struct Bufs {
     int n_scan_tiles = 0;        // prefix-scan tiles (kScanTile bodies each)
     uint64_t *keys[2] = {};      // double-buffered sort keys
     int *vals[2] = {};           // double-buffered body indices
-    int *ghist = nullptr;        // [kPasses * 256] global digit histograms
-    unsigned *status[2] = {};    // [n_sweep_tiles * 256] look-back words per pass
+    int *ghist = nullptr;        // [kHistInts] global digit histograms of all passes
+    int radix_bits = 8;          // digit width of the radix passes: 8, 9 or 11
+    unsigned *status[2] = {};    // [n_sweep_tiles * 2048] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
     DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
     int *pflag = nullptr;        // [n_scan_tiles] scan look-back flags
@@ -59,8 +60,8 @@ struct Bufs {
     int *step_ctr = nullptr;     // [1] step index into dtab, advanced by k_kick_drag
 };
 
-constexpr int kPasses = 8;       // 8-bit digits over the 63-bit key
-constexpr int kSweepItems = 4;
+constexpr int kHistInts = 6 * 2048;  // histograms of the widest setting (6 x 11-bit)
+constexpr int kSweepItems = 8;
 constexpr int kSweepTile = kBlock * kSweepItems;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -124,6 +125,7 @@ void launch_fill_float(float *p, float v, int n, hipStream_t st);
 struct WalkArgs {
     float theta2, eps2, softening, star_mass;
     int quad;                     // 1: add the quadrupole of accepted internal records
+    int targets;                  // targets per wavefront: 64, 32, 16 or 8
     float *ax, *ay, *az;          // original order, written with '='
     float *sax, *say, *saz;       // optional self-gravity outputs
 };
@@ -131,9 +133,11 @@ struct WalkArgs {
 // non-null (graph-replayable), otherwise no perturber.
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
                  Bufs &B, hipStream_t st);
-// Per-body accepted / opened record counts of the gravity walk.
-void launch_walk_counts(const SegTable &T, float theta2, int *n_acc, int *n_open, Bufs &B,
-                        hipStream_t st);
+// Per-body accepted / opened record counts of the gravity walk with the given
+// targets per wave; *popped (optional) accumulates the records the waves pop.
+void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
+                        unsigned long long *popped, Bufs &B, hipStream_t st);
+int walk_waves(const SegTable &T, int targets);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,
                     const float *vz, float restitution, Bufs &B, hipStream_t st);

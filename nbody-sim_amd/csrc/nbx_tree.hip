@@ -422,20 +422,30 @@ __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__rest
 // ===========================================================================
 // Keys: the octant digits of BHTree.cu:164-170 along the float descent of
 // BHTree.cu:253-256, 21 levels, first level most significant.  The same
-// pass accumulates the digit histograms of all radix passes and clears the
-// look-back words of the first pass and of the prefix scan.
+// pass accumulates the digit histograms of all radix passes (BITS-bit
+// digits) and clears the look-back words of the first pass and of the
+// prefix scans.  Each block covers kKeysPerBlock bodies (8 per thread) so
+// that few blocks post histograms.
 // ===========================================================================
+constexpr int kKeysItems = 8;
+constexpr int kKeysPerBlock = kBlock * kKeysItems;
+
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__restrict__ x,
                                                  const float *__restrict__ y,
                                                  const float *__restrict__ z,
                                                  const Root *__restrict__ roots,
-                                                 uint64_t *keys, int *vals, int *ghist,
-                                                 unsigned *status0, int n_status, int *pflag,
-                                                 int n_pflag, int *pflag2, int *tickets)
+                                                 uint64_t *__restrict__ keys,
+                                                 int *__restrict__ vals, int *__restrict__ ghist,
+                                                 unsigned *__restrict__ status0, int n_status,
+                                                 int *__restrict__ pflag, int n_pflag,
+                                                 int *__restrict__ pflag2,
+                                                 int *__restrict__ tickets)
 {
 #pragma clang fp contract(off)
-    __shared__ int h[kPasses * 256];
-    for (int q = threadIdx.x; q < kPasses * 256; q += kBlock) h[q] = 0;
+    constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
+    __shared__ int h[NP * R];
+    for (int q = threadIdx.x; q < NP * R; q += kBlock) h[q] = 0;
     const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
     for (int q = gid; q < n_pflag; q += gsz) {
@@ -444,17 +454,25 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
     }
     if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
     __syncthreads();
-    const int b = blockIdx.x;
-    const int s = seg_of_block(T, b);
-    const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
-    if (i < T.off[s + 1]) {
-        const Root R = roots[s];
-        float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
-        const float px = x[i], py = y[i], pz = z[i];
+    const int n = T.n_total;
+    float px[kKeysItems], py[kKeysItems], pz[kKeysItems];
+#pragma unroll
+    for (int r = 0; r < kKeysItems; ++r) {  // all loads first: one memory latency
+        const int i = blockIdx.x * kKeysPerBlock + r * kBlock + threadIdx.x;
+        px[r] = i < n ? x[i] : 0.f;
+        py[r] = i < n ? y[i] : 0.f;
+        pz[r] = i < n ? z[i] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < kKeysItems; ++r) {
+        const int i = blockIdx.x * kKeysPerBlock + r * kBlock + threadIdx.x;
+        if (i >= n) break;
+        const Root R0 = roots[seg_of_body(T, i)];
+        float cx = R0.c[0], cy = R0.c[1], cz = R0.c[2], hh = R0.c[3];
         uint64_t key = 0;
 #pragma unroll
         for (int l = 0; l < kLevels; ++l) {
-            const bool ox = px > cx, oy = py > cy, oz = pz > cz;
+            const bool ox = px[r] > cx, oy = py[r] > cy, oz = pz[r] > cz;
             key = (key << 3) | (uint64_t)((ox ? 1 : 0) | (oy ? 2 : 0) | (oz ? 4 : 0));
             hh = hh * 0.5f;
             cx = cx + (ox ? hh : -hh);
@@ -464,27 +482,26 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
         keys[i] = key;
         vals[i] = i;
 #pragma unroll
-        for (int p = 0; p < kPasses; ++p) atomicAdd(&h[p * 256 + (int)((key >> (8 * p)) & 255u)], 1);
+        for (int p = 0; p < NP; ++p)
+            atomicAdd(&h[p * R + (int)((key >> (BITS * p)) & (uint64_t)(R - 1))], 1);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < kPasses * 256; q += kBlock)
+    for (int q = threadIdx.x; q < NP * R; q += kBlock)
         if (h[q]) atomicAdd(&ghist[q], h[q]);
 }
 
 // ===========================================================================
-// One stable radix pass (8-bit digit) with decoupled look-back: tiles take
-// tickets in launch order, rank their keys locally (wave match + LDS), post
-// their per-digit counts, and add the counts of earlier tiles read back
-// through one status word per (tile, digit): bit 31 inclusive prefix,
-// bit 30 tile aggregate, low 30 bits the count.
-// mode 0: digit = (key >> 8 pass) & 255;  mode 1: digit = disk of the body.
+// One stable radix pass (BITS-bit digit) with decoupled look-back.  A tile
+// of kSweepTile keys takes a ticket in launch order; each of its four waves
+// ranks its own contiguous quarter in rounds of 64 keys (wave match with
+// BITS ballots, per-wave running digit counters in LDS, no block barrier);
+// the waves' counts are then combined per digit, posted, and the counts of
+// earlier tiles are read back through one status word per (tile, digit):
+// bit 31 inclusive prefix, bit 30 tile aggregate, low 30 bits the count.
+// mode 0: digit = key bits [BITS pass, BITS pass + BITS);
+// mode 1: digit = disk of the body (the final pass of a batched sort).
 // ===========================================================================
 constexpr unsigned kAgg = 1u << 30, kInc = 1u << 31, kValMask = kAgg - 1;
-
-__device__ inline int sweep_digit(int mode, uint64_t key, int val, int shift, const SegTable &T)
-{
-    return mode == 0 ? (int)((key >> shift) & 255u) : seg_of_body(T, val);
-}
 
 // Exclusive scan over the 256 threads of a block; returns the exclusive
 // prefix of v, *total receives the sum.
@@ -512,100 +529,157 @@ __device__ inline int block_excl_scan256(int v, int *total)
     return before + inc - v;
 }
 
+
+template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_sweep(
-    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *kout, int *vout,
-    int n, int pass, int mode, SegTable T, const int *__restrict__ ghist, unsigned *status,
-    unsigned *status_next, int *tickets, int gather, const float *__restrict__ x,
-    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ m,
-    float4 *bodies)
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin, uint64_t *__restrict__ kout,
+    int *__restrict__ vout, int n, int pass, int mode, SegTable T, const int *__restrict__ ghist,
+    unsigned *status, unsigned *status_next, int *tickets, int gather,
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ m, float4 *__restrict__ bodies)
 {
-    __shared__ int s_goff[256], s_cnt[256], s_pre[256];
-    __shared__ int s_wc[kBlock / kWave][256];
+    constexpr int R = 1 << BITS;
+    constexpr int DPT = R / kBlock;              // digits owned by each thread (>= 1)
+    constexpr int ROUNDS = kSweepTile / kBlock;  // rounds of 64 keys per wave
+    // look-back window (tiles per round of independent loads): all tiles post
+    // their aggregates at about the same time, so a wide window resolves the
+    // start-up chain in a few memory round trips
+    constexpr int LB = DPT == 1 ? 32 : (DPT == 2 ? 16 : 4);
+    static_assert(DPT >= 1 && R % kBlock == 0, "radix must be a multiple of the block");
+    __shared__ int s_cnt[kBlock / kWave][R];
+    __shared__ int s_goff[R];
     __shared__ int s_tile;
-    const int t = threadIdx.x, w = t >> 6;
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
     if (t == 0) s_tile = atomicAdd(&tickets[kTicketPass + pass], 1);
-    const int g = mode == 0 ? ghist[pass * 256 + t]
-                            : (t < T.nseg ? T.off[t + 1] - T.off[t] : 0);
+    // global exclusive digit offsets: thread t owns digits [t DPT, (t+1) DPT)
+    int g[DPT];
+    int gs = 0;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        g[q] = mode == 0 ? ghist[pass * R + d] : (d < T.nseg ? T.off[d + 1] - T.off[d] : 0);
+        gs += g[q];
+    }
     int total;
-    s_goff[t] = block_excl_scan256(g, &total);
-    s_cnt[t] = 0;
+    int run = block_excl_scan256(gs, &total);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        s_goff[t * DPT + q] = run;
+        run += g[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; ++q)
+#pragma unroll
+        for (int u = 0; u < DPT; ++u) s_cnt[q][t * DPT + u] = 0;
     __syncthreads();
     const int tile = s_tile;
-    if (status_next) status_next[tile * 256 + t] = 0;
-    const int shift = 8 * pass;
-    const int base = tile * kSweepTile;
-    uint64_t kk[kSweepItems];
-    int vv[kSweepItems], dd[kSweepItems], lo[kSweepItems];
+    if (status_next)
 #pragma unroll
-    for (int r = 0; r < kSweepItems; ++r) {
+        for (int q = 0; q < DPT; ++q) status_next[tile * R + t * DPT + q] = 0;
+    const int shift = BITS * pass;
+    const int base = tile * kSweepTile + w * (kSweepTile / (kBlock / kWave));
+    uint64_t kk[ROUNDS];
+    int vv[ROUNDS], dd[ROUNDS], lo[ROUNDS];
+    int *cnt = s_cnt[w];
+    // issue every load of the wave's chunk before ranking: one memory latency
 #pragma unroll
-        for (int q = 0; q < kBlock / kWave; ++q) s_wc[q][t] = 0;
-        __syncthreads();
-        const int i = base + r * kBlock + t;
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int i = base + r * kWave + lane;
+        kk[r] = i < n ? kin[i] : 0ull;
+        vv[r] = i < n ? vin[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int i = base + r * kWave + lane;
         const bool valid = i < n;
-        uint64_t key = 0;
-        int val = 0, d = 0;
-        if (valid) {
-            key = kin[i];
-            val = vin[i];
-            d = sweep_digit(mode, key, val, shift, T);
-        }
+        const uint64_t key = kk[r];
+        const int val = vv[r];
+        int d = 0;
+        if (valid) d = mode == 0 ? (int)((key >> shift) & (uint64_t)(R - 1)) : seg_of_body(T, val);
         uint64_t peers = ballot64(valid);
 #pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
+        for (int bit = 0; bit < BITS; ++bit) {
             const bool set = (d >> bit) & 1;
             const uint64_t bb = ballot64(set);
             peers &= set ? bb : ~bb;
         }
         const int rank = __popcll(peers & lanemask_lt());
-        if (valid && rank == 0) s_wc[w][d] = __popcll(peers);
-        __syncthreads();
-        {
-            const int c0 = s_wc[0][t], c1 = s_wc[1][t], c2 = s_wc[2][t], c3 = s_wc[3][t];
-            s_wc[0][t] = 0;
-            s_wc[1][t] = c0;
-            s_wc[2][t] = c0 + c1;
-            s_wc[3][t] = c0 + c1 + c2;
-            total = c0 + c1 + c2 + c3;
-        }
-        __syncthreads();
-        kk[r] = key;
-        vv[r] = val;
+        const int before = cnt[d];  // wave-private counter: program order suffices
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) cnt[d] = before + __popcll(peers);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         dd[r] = valid ? d : -1;
-        lo[r] = valid ? s_cnt[d] + s_wc[w][d] + rank : 0;
-        __syncthreads();
-        s_cnt[t] += total;
+        lo[r] = before + rank;
     }
     __syncthreads();
-    // publish this tile's count of digit t and look back over earlier tiles
-    const unsigned cnt = (unsigned)s_cnt[t];
-    unsigned *mine = status + tile * 256 + t;
-    unsigned pre = 0;
-    if (tile == 0) {
-        __hip_atomic_store(mine, kInc | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(mine, kAgg | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int j = tile - 1;
-        while (true) {
-            const unsigned v = __hip_atomic_load(status + j * 256 + t, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            if (v == 0) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            pre += v & kValMask;
-            if (v & kInc) break;
-            --j;
-        }
-        __hip_atomic_store(mine, kInc | (pre + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // per digit: exclusive offsets of the waves inside the tile, tile count
+    unsigned tc[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int d = t * DPT + q;
+        const int c0 = s_cnt[0][d], c1 = s_cnt[1][d], c2 = s_cnt[2][d], c3 = s_cnt[3][d];
+        s_cnt[0][d] = 0;
+        s_cnt[1][d] = c0;
+        s_cnt[2][d] = c0 + c1;
+        s_cnt[3][d] = c0 + c1 + c2;
+        tc[q] = (unsigned)(c0 + c1 + c2 + c3);
     }
-    s_pre[t] = (int)pre;
+    // post this tile's counts, then look back over earlier tiles, LB tiles
+    // and all owned digits per round of independent loads
+    unsigned pre[DPT];
+    bool done[DPT];
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        pre[q] = 0;
+        done[q] = tile == 0;
+        __hip_atomic_store(status + tile * R + t * DPT + q, (tile == 0 ? kInc : kAgg) | tc[q],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (int j = tile - 1; j >= 0; j -= LB) {
+        bool all = true;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) all = all && done[q];
+        if (all) break;
+        unsigned v[DPT][LB];
+#pragma unroll
+        for (int q = 0; q < DPT; ++q)
+#pragma unroll
+            for (int u = 0; u < LB; ++u)
+                v[q][u] = (!done[q] && j - u >= 0)
+                              ? __hip_atomic_load(status + (j - u) * R + t * DPT + q,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : kInc;
+#pragma unroll
+        for (int q = 0; q < DPT; ++q) {
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                if (done[q]) break;
+                unsigned val = v[q][u];
+                while (val == 0) {  // not posted yet: wait for that tile
+                    __builtin_amdgcn_s_sleep(1);
+                    val = __hip_atomic_load(status + (j - u) * R + t * DPT + q, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+                }
+                pre[q] += val & kValMask;
+                if (val & kInc) done[q] = true;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        if (tile > 0)
+            __hip_atomic_store(status + tile * R + t * DPT + q, kInc | (pre[q] + tc[q]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_goff[t * DPT + q] += (int)pre[q];
+    }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSweepItems; ++r) {
+    for (int r = 0; r < ROUNDS; ++r) {
         const int d = dd[r];
         if (d < 0) continue;
-        const int pos = s_goff[d] + s_pre[d] + lo[r];
+        const int pos = s_goff[d] + s_cnt[w][d] + lo[r];
         kout[pos] = kk[r];
         vout[pos] = vv[r];
         if (gather) {
@@ -649,8 +723,13 @@ __device__ inline DVec<N> dv_shfl_up(const DVec<N> &a, int o)
     return r;
 }
 
+// Disk s owns prefix entries [off[s] + s, off[s+1] + s]: its own zero and
+// its n_s running sums, so every disk's sums start from zero exactly as in a
+// plain run of that disk alone.
+__host__ __device__ inline int pbase(const SegTable &T, int s) { return T.off[s] + s; }
+
 template <int N, int KIND>
-__global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bodies, int n,
+__global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__restrict__ bodies,
                                                    DVec<N> *P, int *pflag, DVec<N> *pagg,
                                                    DVec<N> *pinc, int *tickets, int slot,
                                                    int *ghist)
@@ -662,15 +741,18 @@ __global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bo
     if (t == 0) s_tile = atomicAdd(&tickets[slot], 1);
     __syncthreads();
     const int tile = s_tile;
-    const int base = tile * kScanTile + t * kScanItems;
+    const int s = seg_of_stile(T, tile);
+    const int first_tile = T.stile[s];
+    const int ns = T.off[s + 1] - T.off[s];
+    const int lbase = (tile - first_tile) * kScanTile + t * kScanItems;  // disk-local
     DVec<N> v[kScanItems];
     DVec<N> sum = dv_zero<N>();
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int i = base + k;
+        const int i = lbase + k;
         DVec<N> e = dv_zero<N>();
-        if (i < n) {
-            const float4 b = bodies[i];
+        if (i < ns) {
+            const float4 b = bodies[T.off[s] + i];
             const double mm = b.w, x = b.x, y = b.y, z = b.z;
             if (KIND == 0) {
                 e.v[0] = mm;
@@ -705,43 +787,62 @@ __global__ __launch_bounds__(kBlock) void k_prefix(const float4 *__restrict__ bo
         if (k < w) before = dv_add(before, s_w[k]);
         agg = dv_add(agg, s_w[k]);
     }
-    if (t == 0) {
-        DVec<N> pre = dv_zero<N>();
-        if (tile == 0) {
-            pinc[0] = agg;
-            __hip_atomic_store(&pflag[0], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            pagg[tile] = agg;
-            __hip_atomic_store(&pflag[tile], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int j = tile - 1;
-            while (true) {
-                const int f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-                if (f == 0) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                if (f == 2) {
-                    pre = dv_add(pre, pinc[j]);
-                    break;
-                }
-                pre = dv_add(pre, pagg[j]);
-                --j;
+    if (w == 0) {
+        // publish, then look back over up to 64 earlier tiles of this disk at
+        // once: add aggregates up to the nearest inclusive prefix
+        if (tile == first_tile) {
+            if (lane == 0) {
+                pinc[tile] = agg;
+                __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
-            pinc[tile] = dv_add(pre, agg);
-            __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) s_pre = dv_zero<N>();
+        } else {
+            if (lane == 0) {
+                pagg[tile] = agg;
+                __hip_atomic_store(&pflag[tile], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            DVec<N> pre = dv_zero<N>();
+            int j0 = tile - 1;
+            while (true) {
+                const int j = j0 - lane;
+                int f = 2;  // before the disk's first tile: nothing to add
+                if (j >= first_tile) {
+                    f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    while (f == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                const uint64_t incm = ballot64(f == 2);
+                const int stop = incm ? __ffsll((long long)incm) - 1 : 64;  // nearest inclusive
+                DVec<N> c = dv_zero<N>();
+                if (lane < stop) c = pagg[j];
+                else if (lane == stop && j >= first_tile) c = pinc[j];
+#pragma unroll
+                for (int q = 0; q < N; ++q)
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) c.v[q] += __shfl_xor(c.v[q], o, 64);
+                pre = dv_add(pre, c);
+                if (incm) break;
+                j0 -= 64;
+            }
+            if (lane == 0) {
+                pinc[tile] = dv_add(pre, agg);
+                __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                s_pre = pre;
+            }
         }
-        s_pre = pre;
-        if (tile == 0) P[0] = dv_zero<N>();
+        if (tile == first_tile && lane == 0) P[pbase(T, s)] = dv_zero<N>();
     }
     if (KIND == 0 && tile == 0)  // histograms are consumed: clear them for the next build
-        for (int q = t; q < kPasses * 256; q += kBlock) ghist[q] = 0;
+        for (int q = t; q < kHistInts; q += kBlock) ghist[q] = 0;
     __syncthreads();
     DVec<N> run = dv_add(s_pre, dv_add(before, excl));
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         run = dv_add(run, v[k]);
-        if (base + k < n) P[base + k + 1] = run;
+        if (lbase + k < ns) P[pbase(T, s) + lbase + k + 1] = run;
     }
 }
 
@@ -804,7 +905,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     const int dpar = max(lcp(k, n, lo - 1, lo), lcp(k, n, hi, hi + 1));
     octf[j] = (dpar < 0 || oct_level(dpar) != lev || lev == kLevels) ? 1 : 0;
     // mass and COM over sorted positions [lo, hi]
-    const DVec<4> a = P[T.off[s] + lo], b = P[T.off[s] + hi + 1];
+    const DVec<4> a = P[pbase(T, s) + lo], b = P[pbase(T, s) + hi + 1];
     const double mm = b.v[0] - a.v[0];
     double cxd = 0, cyd = 0, czd = 0;
     float4 c;
@@ -821,7 +922,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     if (P2) {
         // traceless quadrupole about the COM: Q = 3 C - tr(C) I with the
         // central second moment C = S - M c c^T of the range sums S
-        const DVec<6> u = P2[T.off[s] + lo], o = P2[T.off[s] + hi + 1];
+        const DVec<6> u = P2[pbase(T, s) + lo], o = P2[pbase(T, s) + hi + 1];
         const double sxx = o.v[0] - u.v[0] - mm * cxd * cxd;
         const double syy = o.v[1] - u.v[1] - mm * cyd * cyd;
         const double szz = o.v[2] - u.v[2] - mm * czd * czd;
@@ -1078,32 +1179,44 @@ void launch_bbox(const SegTable &T, const float *x, const float *y, const float 
 
 void launch_root(const SegTable &, Bufs &, hipStream_t) {}
 
+template <int BITS>
+static void launch_sort(const SegTable &T, const float *x, const float *y, const float *z,
+                        const float *mass, Bufs &B, hipStream_t st)
+{
+    constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
+    const int n = T.n_total;
+    hipLaunchKernelGGL(k_keys<BITS>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
+                       z, B.roots, B.keys[0], B.vals[0], B.ghist, B.status[0],
+                       R * B.n_sweep_tiles, B.pflag, B.n_scan_tiles, B.pflag2, B.tickets);
+    const int npass = NP + (T.nseg > 1 ? 1 : 0);
+    int cur = 0;
+    for (int p = 0; p < npass; ++p) {
+        const bool last = p == npass - 1;
+        const int mode = p < NP ? 0 : 1;
+        hipLaunchKernelGGL(k_sweep<BITS>, dim3(B.n_sweep_tiles), dim3(kBlock), 0, st, B.keys[cur],
+                           B.vals[cur], B.keys[cur ^ 1], B.vals[cur ^ 1], n, p, mode, T, B.ghist,
+                           B.status[p & 1], last ? nullptr : B.status[(p + 1) & 1], B.tickets,
+                           last ? 1 : 0, x, y, z, mass, B.bodies);
+        cur ^= 1;
+    }
+    B.cur = cur;
+}
+
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
                   const float *mass, Bufs &B, hipStream_t st)
 {
     const int n = T.n_total;
-    const int nst = B.n_sweep_tiles * 256;
-    hipLaunchKernelGGL(k_keys, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.roots,
-                       B.keys[0], B.vals[0], B.ghist, B.status[0], nst, B.pflag,
-                       B.n_scan_tiles, B.pflag2, B.tickets);
-    const int npass = kPasses + (T.nseg > 1 ? 1 : 0);
-    int cur = 0;
-    for (int p = 0; p < npass; ++p) {
-        const bool last = p == npass - 1;
-        const int mode = p < kPasses ? 0 : 1;
-        hipLaunchKernelGGL(k_sweep, dim3(B.n_sweep_tiles), dim3(kBlock), 0, st, B.keys[cur],
-                           B.vals[cur], B.keys[cur ^ 1], B.vals[cur ^ 1], n, p, mode, T,
-                           B.ghist, B.status[p & 1], last ? nullptr : B.status[(p + 1) & 1],
-                           B.tickets, last ? 1 : 0, x, y, z, mass, B.bodies);
-        cur ^= 1;
+    switch (B.radix_bits) {
+    case 9: launch_sort<9>(T, x, y, z, mass, B, st); break;
+    case 11: launch_sort<11>(T, x, y, z, mass, B, st); break;
+    default: launch_sort<8>(T, x, y, z, mass, B, st); break;
     }
-    B.cur = cur;
-    hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies, n,
+    hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T, B.bodies,
                        B.prefix, B.pflag, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist);
     if (B.quad)
-        hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, B.bodies,
-                           n, B.prefix2, B.pflag2, B.pagg2, B.pinc2, B.tickets, kTicketScan2,
-                           B.ghist);
+        hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
+                           B.bodies, B.prefix2, B.pflag2, B.pagg2, B.pinc2, B.tickets,
+                           kTicketScan2, B.ghist);
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
         hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[B.cur], B.prefix,

@@ -24,7 +24,7 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 constexpr int kWalkWaves = 1;   // one wavefront per block: walks never synchronise
 constexpr int kWalkBlock = kWave * kWalkWaves;
-constexpr int kGStack = 1024;   // gravity stack entries per wave
+constexpr int kGStack = 256;    // gravity stack entries per wave (deepest measured: 75)
 constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
 
 struct StackEntry {
@@ -50,18 +50,26 @@ struct Rec {
     float4 q0, q1;
 };
 
-// Visitor: visit(rec, leaf) for one record on the lanes that own it;
-// returns true when the lane opens the (internal) record.
-template <bool QUAD, class Lds, class Visitor>
+// Wave-level walk over TG targets.  Lane = r * TG + t serves target t and
+// record slot r (L = 64 / TG slots): each popped batch of up to 64 records is
+// evaluated L records at a time, lane (t, r) taking record k0 + r for target
+// t.  Stack masks carry one bit per target.  With TG = 64 (L = 1) every lane
+// owns one target and evaluates every record of the batch.
+// Visitor: visit(rec, leaf) on the lanes that own the record's bit; returns
+// true when the lane opens the (internal) record.
+template <int TG, bool QUAD, class Lds, class Visitor>
 __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
                                  const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
-                                 int *err, Visitor &vis)
+                                 int *err, Visitor &vis, unsigned long long *popped = nullptr)
 {
+    constexpr int LS = kWave / TG;
+    constexpr uint64_t TMASK = TG == 64 ? ~0ull : ((1ull << TG) - 1ull);
     const int lane = __lane_id();
+    const int tt = lane % TG, rr = lane / TG;
     // every lane writes the same entry: per-lane program order covers the
     // later reads, no cross-lane ordering is needed
     L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
-    int sp = 1;
+    int sp = 1, sp_max = 1;
     while (sp > 0) {
         // take up to 64 records from the top blocks (from the end of a block)
         int nrec = 0, my_rec = -1;
@@ -81,7 +89,8 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
             }
             nrec += take;
         }
-        // one vector load per lane, staged in LDS for broadcast reads
+        if (popped && lane == 0) atomicAdd(popped, (unsigned long long)nrec);
+        // one vector load per lane, staged in LDS
         if (my_rec >= 0) {
             L.a[lane] = ra[my_rec];
             L.b[lane] = rb[my_rec];
@@ -94,7 +103,10 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k = 0; k < nrec; ++k) {
+        for (int k0 = 0; k0 < nrec; k0 += LS) {
+            // slots past the batch read a valid staging entry and are masked off
+            const int k = min(k0 + rr, nrec - 1);
+            const bool mine = (k0 + rr < nrec) && ((L.m[k] >> tt) & 1ull);
             Rec r;
             r.a = L.a[k];
             r.b = L.b[k];
@@ -102,24 +114,46 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
                 r.q0 = L.q[2 * k];
                 r.q1 = L.q[2 * k + 1];
             }
-            const int4 b = r.b;
-            const uint64_t mk = L.m[k];
-            const bool leaf = b.w == 0;
-            const bool open = ((mk >> lane) & 1ull) && vis.visit(r, leaf);
+            const bool open = vis.visit_masked(r, mine);
             const uint64_t om = ballot64(open);
             if (om) {
-                if (sp < kGStack) {
-                    L.st[sp] = StackEntry{b.z, b.w, (unsigned)om, (unsigned)(om >> 32)};
-                    ++sp;
-                } else if (lane == 0) {
-                    atomicOr(err, kErrStack);
+#pragma unroll
+                for (int q = 0; q < LS; ++q) {
+                    const uint64_t mq = (om >> (q * TG)) & TMASK;
+                    if (mq) {
+                        const int4 b = L.b[k0 + q];
+                        if (sp < kGStack) {
+                            L.st[sp] = StackEntry{b.z, b.w, (unsigned)mq, (unsigned)(mq >> 32)};
+                            ++sp;
+                        } else if (lane == 0) {
+                            atomicOr(err, kErrStack);
+                        }
+                    }
                 }
+                sp_max = max(sp_max, sp);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if (lane == 0) atomicMax(err + 2, sp_max);  // deepest stack (diagnostic)
+}
+
+// Sum of a per-lane value over the LS record slots of target tt.
+template <int TG>
+__device__ inline float slot_sum(float v)
+{
+#pragma unroll
+    for (int o = TG; o < kWave; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <int TG>
+__device__ inline int slot_sum_i(int v)
+{
+#pragma unroll
+    for (int o = TG; o < kWave; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -132,33 +166,35 @@ struct GravityVisitor {
     float bx, by, bz, theta2, eps2;
     int self;
     float fx, fy, fz;
-    __device__ inline bool visit(const Rec &r, bool leaf)
+    // Branch-free: every lane evaluates the record and selects the result, so
+    // the wave never splits its execution mask inside the record loop.
+    __device__ inline bool visit_masked(const Rec &r, bool mine)
     {
         const float4 &a = r.a;
         const int4 &b = r.b;
+        const bool leaf = b.w == 0;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
-        if (!open && !(leaf && b.z == self)) {
-            const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
-            const float rinv2 = rinv * rinv;
-            const float rinv3 = rinv2 * rinv;
-            const float f = a.w * rinv3;
-            fx += f * dx;
-            fy += f * dy;
-            fz += f * dz;
-            if constexpr (QUAD) {
-                // a += -Q R / r^5 + (5/2) (R.Q R) R / r^7 for R = COM - body
-                // (Q traceless; zero for leaves)
-                const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
-                const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
-                const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
-                const float rinv5 = rinv3 * rinv2;
-                const float g = 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2;
-                fx += g * dx - rinv5 * qx;
-                fy += g * dy - rinv5 * qy;
-                fz += g * dz - rinv5 * qz;
-            }
+        const bool open = mine && !leaf && !(__int_as_float(b.y) < theta2 * r2);
+        const bool take = mine && !open && !(leaf && b.z == self);
+        const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
+        const float rinv2 = rinv * rinv;
+        const float rinv3 = rinv2 * rinv;
+        const float f = take ? a.w * rinv3 : 0.f;
+        fx += f * dx;
+        fy += f * dy;
+        fz += f * dz;
+        if constexpr (QUAD) {
+            // a += -Q R / r^5 + (5/2) (R.Q R) R / r^7 for R = COM - body
+            // (Q traceless; zero for leaves)
+            const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
+            const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
+            const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
+            const float rinv5 = take ? rinv3 * rinv2 : 0.f;
+            const float g = 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2;
+            fx += g * dx - rinv5 * qx;
+            fy += g * dy - rinv5 * qy;
+            fz += g * dz - rinv5 * qz;
         }
         return open;
     }
@@ -170,6 +206,10 @@ struct CountVisitor {
     float bx, by, bz, theta2;
     int self;
     int n_acc, n_open;
+    __device__ inline bool visit_masked(const Rec &r, bool mine)
+    {
+        return mine && visit(r, r.b.w == 0);
+    }
     __device__ inline bool visit(const Rec &r, bool leaf)
     {
         const float4 &a = r.a;
@@ -183,9 +223,21 @@ struct CountVisitor {
     }
 };
 
-template <bool QUAD>
+// Waves per disk for a given number of targets per wave.
+struct WaveMap {
+    int wave[kMaxSeg + 1];
+};
+
+__device__ inline int seg_of_wmap(const SegTable &T, const WaveMap &M, int w)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && w >= M.wave[s + 1]) ++s;
+    return s;
+}
+
+template <int TG, bool QUAD>
 __global__ __launch_bounds__(kWalkBlock) void k_walk(
-    SegTable T, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
+    SegTable T, WaveMap M, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
     const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
@@ -195,14 +247,17 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     __shared__ Lds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
-    if (w >= T.wave[T.nseg]) return;  // uniform per wave
-    const int s = seg_of_wave(T, w);
-    const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
+    if (w >= M.wave[T.nseg]) return;  // uniform per wave
+    const int s = seg_of_wmap(T, M, w);
+    const int lane = __lane_id(), tt = lane % TG, rr = lane / TG;
+    const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
     GravityVisitor<QUAD> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
-    wave_walk<QUAD>(ra, rb, rq, T.noff[s], ballot64(valid), lds[wib], err, v);
-    if (!valid) return;
+    const uint64_t active = ballot64(valid && rr == 0);  // bit tt per valid target
+    wave_walk<TG, QUAD>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
+    const float fx = slot_sum<TG>(v.fx), fy = slot_sum<TG>(v.fy), fz = slot_sum<TG>(v.fz);
+    if (!valid || rr != 0) return;
     const int i = perm[pos];
     float ox, oy, oz;
     {
@@ -212,9 +267,9 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
         float r2 = dx * dx + dy * dy + dz * dz + softening * softening;
         float r = sqrtf(r2);
         float f = star_mass / (r2 * r);
-        ox = f * dx + v.fx;
-        oy = f * dy + v.fy;
-        oz = f * dz + v.fz;
+        ox = f * dx + fx;
+        oy = f * dy + fy;
+        oz = f * dz + fz;
         DiskDev d{};
         if (dtab) d = dtab[*step_ctr * kMaxSeg + s];
         if (d.pmass > 0.f)
@@ -224,33 +279,35 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     ay[i] = oy;
     az[i] = oz;
     if (sax) {
-        sax[i] = v.fx;
-        say[i] = v.fy;
-        saz[i] = v.fz;
+        sax[i] = fx;
+        say[i] = fy;
+        saz[i] = fz;
     }
 }
 
-__global__ __launch_bounds__(kWalkBlock) void k_walk_counts(SegTable T, const float4 *__restrict__ ra,
-                                                        const int4 *__restrict__ rb,
-                                                        const float4 *__restrict__ bodies,
-                                                        const int *__restrict__ perm,
-                                                        float theta2, int *n_acc, int *n_open,
-                                                        int *err)
+template <int TG>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
+    SegTable T, WaveMap M, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
+    const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, int *n_acc,
+    int *n_open, int *err, unsigned long long *popped)
 {
     __shared__ WaveLds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
-    if (w >= T.wave[T.nseg]) return;
-    const int s = seg_of_wave(T, w);
-    const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
+    if (w >= M.wave[T.nseg]) return;
+    const int s = seg_of_wmap(T, M, w);
+    const int lane = __lane_id(), tt = lane % TG, rr = lane / TG;
+    const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
     CountVisitor v{b.x, b.y, b.z, theta2, pos, 0, 0};
-    wave_walk<false>(ra, rb, nullptr, T.noff[s], ballot64(valid), lds[wib], err, v);
-    if (!valid) return;
+    wave_walk<TG, false>(ra, rb, nullptr, T.noff[s], ballot64(valid && rr == 0), lds[wib], err, v,
+                         popped);
+    const int na = slot_sum_i<TG>(v.n_acc), no = slot_sum_i<TG>(v.n_open);
+    if (!valid || rr != 0) return;
     const int i = perm[pos];
-    n_acc[i] = v.n_acc;
-    n_open[i] = v.n_open;
+    if (n_acc) n_acc[i] = na;
+    if (n_open) n_open[i] = no;
 }
 
 // ---------------------------------------------------------------------------
@@ -477,22 +534,62 @@ __global__ void k_fof_final(int *parent, int *labels, int n)
 // Launchers
 // ===========================================================================
 
+template <int TG>
+static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
+                           const WalkArgs &W, Bufs &B, hipStream_t st)
+{
+    WaveMap M;
+    M.wave[0] = 0;
+    for (int s = 0; s < kMaxSeg; ++s)
+        M.wave[s + 1] = M.wave[s] + (s < T.nseg ? cdiv(T.off[s + 1] - T.off[s], TG) : 0);
+    const int waves = M.wave[T.nseg];
+    auto kern = W.quad ? k_walk<TG, true> : k_walk<TG, false>;
+    hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, M, dtab,
+                       step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
+                       B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay,
+                       W.az, W.sax, W.say, W.saz, B.err);
+}
+
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
                  Bufs &B, hipStream_t st)
 {
-    const int waves = T.wave[T.nseg];
-    auto kern = W.quad ? k_walk<true> : k_walk<false>;
-    hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, dtab,
-                       step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies, B.vals[B.cur], W.theta2, W.eps2, W.softening,
-                       W.star_mass, W.ax, W.ay, W.az, W.sax, W.say, W.saz, B.err);
+    switch (W.targets) {
+    case 8: launch_walk_tg<8>(T, dtab, step_ctr, W, B, st); break;
+    case 16: launch_walk_tg<16>(T, dtab, step_ctr, W, B, st); break;
+    case 32: launch_walk_tg<32>(T, dtab, step_ctr, W, B, st); break;
+    default: launch_walk_tg<64>(T, dtab, step_ctr, W, B, st); break;
+    }
 }
 
-void launch_walk_counts(const SegTable &T, float theta2, int *n_acc, int *n_open, Bufs &B,
-                        hipStream_t st)
+template <int TG>
+static void launch_counts_tg(const SegTable &T, float theta2, int *n_acc, int *n_open,
+                             unsigned long long *popped, Bufs &B, hipStream_t st)
 {
-    const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_walk_counts, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
-                       B.rec_a, B.rec_b, B.bodies, B.vals[B.cur], theta2, n_acc, n_open, B.err);
+    WaveMap M;
+    M.wave[0] = 0;
+    for (int s = 0; s < kMaxSeg; ++s)
+        M.wave[s + 1] = M.wave[s] + (s < T.nseg ? cdiv(T.off[s + 1] - T.off[s], TG) : 0);
+    hipLaunchKernelGGL(k_walk_counts<TG>, dim3(cdiv(M.wave[T.nseg], kWalkWaves)),
+                       dim3(kWalkBlock), 0, st, T, M, B.rec_a, B.rec_b, B.bodies, B.vals[B.cur],
+                       theta2, n_acc, n_open, B.err, popped);
+}
+
+int walk_waves(const SegTable &T, int targets)
+{
+    int w = 0;
+    for (int s = 0; s < T.nseg; ++s) w += cdiv(T.off[s + 1] - T.off[s], targets);
+    return w;
+}
+
+void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
+                        unsigned long long *popped, Bufs &B, hipStream_t st)
+{
+    switch (targets) {
+    case 8: launch_counts_tg<8>(T, theta2, n_acc, n_open, popped, B, st); break;
+    case 16: launch_counts_tg<16>(T, theta2, n_acc, n_open, popped, B, st); break;
+    case 32: launch_counts_tg<32>(T, theta2, n_acc, n_open, popped, B, st); break;
+    default: launch_counts_tg<64>(T, theta2, n_acc, n_open, popped, B, st); break;
+    }
 }
 
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,

@@ -79,6 +79,8 @@ EXPORTS = [
     "nbx_reset_acceleration", "nbx_gas_drag", "nbx_fof", "nbx_init_forces", "nbx_step",
     "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_walk_counts",
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
+    "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
+    "nbx_tree_set_radix_bits",
 ]
 
 _lib = None
@@ -116,6 +118,11 @@ def lib():
             "nbx_tree_set_multipole": [vp, i32],
             "nbx_tree_set_precision": [vp, i32],
             "nbx_diagnostics": [vp, P, f32, f32, i32, C.POINTER(C.c_double)],
+            "nbx_tree_set_walk_shape": [vp, i32],
+            "nbx_tree_get_walk_shape": [vp, C.POINTER(i32)],
+            "nbx_tune_walk": [vp, f32, f32, f32, C.POINTER(f32)],
+            "nbx_walk_stats": [vp, f32, C.POINTER(C.c_double)],
+            "nbx_tree_set_radix_bits": [vp, i32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -407,6 +414,13 @@ def walk_counts(t, p, theta):
     return float(a.double().mean()), float(o.double().mean())
 
 
+def walk_stats(t, theta):
+    """{acc, open: mean per body; records_per_wave; max_stack} of the theta walk."""
+    out = (C.c_double * 4)()
+    _check(lib().nbx_walk_stats(t.h, theta, out), "walk_stats")
+    return dict(acc=out[0], open=out[1], records_per_wave=out[2], max_stack=out[3])
+
+
 def fof_into(t, p, link_length, labels):
     """FoF labels into an existing int32 device tensor (no allocation)."""
     _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, labels.data_ptr()), "fof")
@@ -507,6 +521,28 @@ def diagnostics(t, p, star_mass=1.0, softening=1e-3, with_self=False):
     return dict(kinetic=out[0], star_potential=out[1], L=(out[2], out[3], out[4]),
                 self_potential=out[5], mass=out[6],
                 energy=out[0] + out[1] + out[5])
+
+
+def set_radix_bits(t, bits):
+    """Digit width of the key sort's radix passes (8, 9 or 11)."""
+    _check(lib().nbx_tree_set_radix_bits(t.h, int(bits)), "set_radix_bits")
+
+
+def set_walk_shape(t, targets_per_wave):
+    _check(lib().nbx_tree_set_walk_shape(t.h, int(targets_per_wave)), "set_walk_shape")
+
+
+def walk_shape(t):
+    v = C.c_int32()
+    _check(lib().nbx_tree_get_walk_shape(t.h, C.byref(v)), "walk_shape")
+    return v.value
+
+
+def tune_walk(t, theta=0.5, softening=1e-3, star_mass=1.0):
+    """Autotune the walk shape on the current tree; returns {targets: ms}."""
+    ms = (C.c_float * 4)()
+    _check(lib().nbx_tune_walk(t.h, theta, softening, star_mass, ms), "tune_walk")
+    return {64: ms[0], 32: ms[1], 16: ms[2], 8: ms[3]}
 
 
 def set_async(t, on):

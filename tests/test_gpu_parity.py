@@ -84,12 +84,15 @@ def test_perturber_kernel_kats(gpu):
 # ---------------------------------------------------------------------------
 # Keys and order: bit-exact against the oracle and against the reference DFS
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("n,seed", [(10, 0), (1000, 1), (4096, 2), (131072, 3)])
-def test_keys_and_order_bit_exact(gpu, n, seed):
+@pytest.mark.parametrize("n,seed,bits", [(10, 0, 8), (1000, 1, 8), (4096, 2, 8), (131072, 3, 8),
+                                         (131072, 3, 9), (131072, 3, 11), (5000, 4, 11)])
+def test_keys_and_order_bit_exact(gpu, n, seed, bits):
     nbx = gpu
     d = disk(n, seed=seed)
     p = nbx.to_gpu(d)
-    t = _build(nbx, p)
+    t = nbx.allocateOctree(n)
+    nbx.set_radix_bits(t, bits)
+    t = _build(nbx, p, t)
     keys = O.mt_keys(d["x"], d["y"], d["z"])
     perm = O.mt_sort(keys)
     g_perm = _np(t.sort_idx())
@@ -143,8 +146,10 @@ def test_single_particle_star_force(gpu):
     assert g["ax"][0] == ref["a"][0, 0]
 
 
-@pytest.mark.parametrize("n,mass,seed", [(4096, 1e-5, 1), (131072, 3e-5, 2)])
-def test_force_vs_oracle(gpu, n, mass, seed):
+@pytest.mark.parametrize("n,mass,seed,shape", [(4096, 1e-5, 1, 64), (131072, 3e-5, 2, 64),
+                                               (131072, 3e-5, 2, 32), (131072, 3e-5, 2, 16),
+                                               (4096, 1e-5, 1, 8)])
+def test_force_vs_oracle(gpu, n, mass, seed, shape):
     """Tree self-gravity vs the oracle's walk of the same compressed octree
     (median rel diff <= 2e-6, 99.9th pct <= 1e-3: COM rounding and the
     rsqrt approximation), total acceleration vs the reference walk restated
@@ -155,6 +160,7 @@ def test_force_vs_oracle(gpu, n, mass, seed):
     d = disk(n, mass=mass, seed=seed)
     p = nbx.to_gpu(d)
     t = _build(nbx, p)
+    nbx.set_walk_shape(t, shape)
     sa = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
     nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0, self_acc=sa)
     g_self = np.stack([_np(s) for s in sa], 1)
@@ -498,3 +504,19 @@ def test_fp64_diagnostics(gpu):
     assert abs(out["star_potential"] - pe) <= 1e-12 * abs(pe)
     assert np.allclose(out["L"], L, rtol=1e-12, atol=1e-20)
     assert abs(out["self_potential"] - ps) <= 1e-10 * abs(ps)
+
+
+def test_walk_autotuner(gpu):
+    """The autotuner times every walk shape on the current tree, keeps the
+    fastest and leaves the particle state untouched."""
+    nbx = gpu
+    d = disk(65536, mass=3e-5, seed=61)
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    before = p.numpy()
+    ms = nbx.tune_walk(t, 0.5, 1e-3, 1.0)
+    assert set(ms) == {64, 32, 16, 8} and all(v > 0 for v in ms.values())
+    assert nbx.walk_shape(t) == min(ms, key=ms.get)
+    after = p.numpy()
+    for k in before:
+        np.testing.assert_array_equal(before[k], after[k])
