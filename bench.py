@@ -28,6 +28,22 @@ DISK_MASS = 3e-5
 FOF_EVERY = 100
 MI355X_HBM_GBPS = 8000.0
 MI355X_FP32_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(_REPO, "profiles", "traffic_c2.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+    counter reads for a plain copy of known size); None when not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) >= 6 and w[0] == "kernel" and w[1] == kernel:
+                    return float(w[3]) + float(w[5])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -43,6 +59,9 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying a hipGraph")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (calibrates the HBM "
+                         "byte counters of a rocprofv3 --pmc run)")
     return ap.parse_args()
 
 
@@ -60,7 +79,7 @@ def cpu_baseline(d, r_coll, pert_mass_mjup, n_steps_hint):
     t0 = time.perf_counter()
     O.ref_steps(s, d["mass"], params, 0, 1, nthreads=threads)
     one = time.perf_counter() - t0
-    steps = n_steps_hint or max(1, min(200, int(15.0 / max(one, 1e-3))))
+    steps = n_steps_hint or max(1, min(1000, int(15.0 / max(one, 1e-3))))
     t0 = time.perf_counter()
     O.ref_steps(s, d["mass"], params, 1, steps, nthreads=threads)
     el = time.perf_counter() - t0
@@ -86,6 +105,13 @@ def main():
     if not torch.cuda.is_available():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calib > 0:
+        src = torch.ones(args.copy_calib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n = args.n
     d = make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024)
     r_coll = nbx.particlePhysicalRadius(np.float32(DISK_MASS / n))
@@ -139,13 +165,15 @@ def main():
     build_bytes = float(n) * (12 + 24 + 8 * 24 + 56 + 32 + 32)
     build_gbps = build_bytes / (phases["build"] * 1e-3) / 1e9 if phases["build"] > 0 else 0.0
     ms_per_step = elapsed / args.steps * 1e3
+    # the walk is the step's dominant kernel; it computes on the FP32 VALU (no
+    # contraction to put on the matrix cores), priced against the FP32 peak
     roof_walk = {"bound": "valu", "achieved": walk_tflops, "peak": MI355X_FP32_TFLOPS,
-                 "unit": "TFLOP/s", "frac": walk_tflops / MI355X_FP32_TFLOPS, "traffic": None,
+                 "unit": "TFLOP/s", "frac": walk_tflops / MI355X_FP32_TFLOPS,
+                 "traffic": measured_traffic(f"k_walk<{nbx.walk_shape(t)},false>"),
                  "kernel": "k_walk", "ms": phases["walk"]}
     roof_build = {"bound": "hbm", "achieved": build_gbps, "peak": MI355X_HBM_GBPS,
                   "unit": "GB/s", "frac": build_gbps / MI355X_HBM_GBPS, "traffic": None,
                   "kernel": "build phase", "ms": phases["build"]}
-    dominant = roof_walk if phases["walk"] >= phases["build"] else roof_build
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -156,8 +184,8 @@ def main():
                                "drag 1e-3, perturber 10 M_Jup @ 5.2 AU, FoF every 100",
                    "n": n, "theta": 0.5, "parallelism": "single GPU",
                    "launch": "direct" if args.no_graph else "hipGraph replay per step"},
-        "roofline": dominant,
-        "roofline_walk": roof_walk,
+        "roofline": roof_walk,
+        "roofline_build": roof_build,
         "phases_ms": phases,
         "walk_shape": {"targets_per_wave": nbx.walk_shape(t),
                        "tuned_ms": {str(k): v for k, v in tune_ms.items()}},
