@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying a hipGraph")
+    ap.add_argument("--no-window", action="store_true",
+                    help="skip the differentiable-window leg")
+    ap.add_argument("--window-steps", type=int, default=200)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run)")
@@ -193,6 +196,8 @@ def main():
     }
     if not args.no_batch:
         out["batched_sweep"] = batched_sweep(nbx, np, torch, make_disk, args)
+    if not args.no_window:
+        out["diff_window"] = diff_window(nbx, np, torch, make_disk, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024), r_coll,
                                            10.0, args.cpu_steps)
@@ -257,6 +262,54 @@ def batched_sweep(nbx, np, torch, make_disk, args):
             "sequential_particle_steps_per_s": total * k / seq_s,
             "speedup_vs_sequential": seq_s / bat_s,
             "batched_ms_per_step": bat_s / k * 1e3, "sequential_ms_per_step": seq_s / k * 1e3}
+
+
+def diff_window(nbx, np, torch, make_disk, args):
+    """Differentiable window: seeded disk of 32,768 bodies (3e-5 M_sun), no
+    collisions, gravity + drag 1e-3 + 10 M_Jup perturber at 5.2 AU, 200
+    steps with a checkpoint every 20; loss = mean e^2 + i^2 of the bodies in
+    1-2 AU at the end.  Times the forward window and forward + reverse, and
+    checks d loss / d(perturber mass, radius) against central differences
+    (relative step 1e-2) of the forward window."""
+    from nbx.window import WindowConfig, annulus_mask, run_window, window, window_loss
+    n = 32768
+    d = make_disk(n, DISK_MASS, 0.5, 4.0, seed=77)
+    x0 = torch.tensor(np.stack([d["x"], d["y"], d["z"]]), device="cuda").contiguous()
+    v0 = torch.tensor(np.stack([d["vx"], d["vy"], d["vz"]]), device="cuda").contiguous()
+    m = torch.tensor(d["mass"], device="cuda")
+    t = nbx.allocateOctree(n)
+    cfg = WindowConfig(n_steps=args.window_steps, checkpoint_every=20)
+    run_window(x0, v0, 10.0, 5.2, m, t, cfg)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xT, vT, _ = run_window(x0, v0, 10.0, 5.2, m, t, cfg)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    mask = annulus_mask(xT)
+    pm = torch.tensor(10.0, dtype=torch.float64, requires_grad=True)
+    rp = torch.tensor(5.2, dtype=torch.float64, requires_grad=True)
+    v0g = v0.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xT, vT = window(x0, v0g, pm, rp, m, t, cfg)
+    L = window_loss(xT, vT, mask)
+    L.backward()
+    torch.cuda.synchronize()
+    fb_ms = (time.perf_counter() - t0) * 1e3
+
+    def loss_at(pmv, rpv):
+        x, v, _ = run_window(x0, v0, pmv, rpv, m, t, cfg)
+        return window_loss(x, v, mask).item()
+
+    h = 1e-2
+    fd_m = (loss_at(10.0 * (1 + h), 5.2) - loss_at(10.0 * (1 - h), 5.2)) / (2 * h * 10.0)
+    fd_r = (loss_at(10.0, 5.2 * (1 + h)) - loss_at(10.0, 5.2 * (1 - h))) / (2 * h * 5.2)
+    return {"n": n, "steps": cfg.n_steps, "checkpoint_every": cfg.checkpoint_every,
+            "bodies_in_loss": int(mask.sum().item()), "loss": L.item(),
+            "forward_ms": fwd_ms, "forward_reverse_ms": fb_ms,
+            "dL_dmass_mjup": pm.grad.item(), "fd_dmass_mjup": fd_m,
+            "dL_dradius_au": rp.grad.item(), "fd_dradius_au": fd_r,
+            "dL_dv0_norm": torch.linalg.norm(v0g.grad.double()).item()}
 
 
 def force_accuracy(nbx, t, p, n, n_targets=1024):

@@ -192,6 +192,35 @@ int nbx_tree_set_precision(nbx_tree *t, int compensated);
 int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float softening,
                     int with_self, double *out);
 
+/* ---- reverse mode (no reference counterpart) ------------------------------
+ * Adjoint of the collision-free step of main.cu:101-140 (KDK, gas drag,
+ * monopole tree walk with the reference's acceptance, star, perturber), for
+ * one disk per tree, float KDK.  The interaction lists are those of the
+ * forward walk on the same positions.  Adjoint arrays are device arrays of
+ * 3 n doubles laid out [x | y | z] in original body order; pgrad is a host
+ * array of 2 doubles that accumulates dL/d(perturber mass in M_sun,
+ * perturber radius in AU).  Calls return after the device work. */
+/* Allocate the reverse-mode buffers; later builds also record the sorted
+ * body range of every walk record. */
+int nbx_tree_set_adjoint(nbx_tree *t, int enable);
+/* xbar += (dA/dx)^T abar for A = tree + star + perturber acceleration at the
+ * positions of p and the perturber at sim_time (rebuilds the tree). */
+int nbx_adj_force(nbx_tree *t, const nbx_particles *p, float theta, float softening,
+                  float star_mass, const nbx_disk_params *dp, float sim_time, const double *abar,
+                  double *xbar, double *pgrad);
+/* Reverse of step `step` (0-based, perturber at (step+1) dt): s0 is the
+ * state (x, v, a) before the step, s1 after it.  On entry xbar, vbar are the
+ * adjoints of x, v of s1 and abar the share of a of s1 from the next step's
+ * first kick; on return they are the adjoints of x, v, a of s0. */
+int nbx_adj_step(nbx_tree *t, const nbx_particles *s0, const nbx_particles *s1,
+                 const nbx_step_params *sp, const nbx_disk_params *dp, int32_t step,
+                 double *xbar, double *vbar, double *abar, double *pgrad);
+/* Reverse of nbx_init_forces (perturber at t = 0): xbar += (dA/dx)^T abar at
+ * the positions of s0. */
+int nbx_adj_init_forces(nbx_tree *t, const nbx_particles *s0, const nbx_step_params *sp,
+                        const nbx_disk_params *dp, const double *abar, double *xbar,
+                        double *pgrad);
+
 /* ---- diagnostics --------------------------------------------------------- */
 /* Tree statistics of the last build: number of octree nodes and maximum
  * depth per disk (host arrays of length n_seg, may be NULL). */

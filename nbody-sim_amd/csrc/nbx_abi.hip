@@ -67,6 +67,8 @@ struct nbx_tree {
     bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
     int walk_tg = kDefaultTargets;  // gravity-walk targets per wavefront
     float tune_ms[4] = {0, 0, 0, 0};  // autotuner timings for 64, 32, 16, 8 targets
+    bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
+    double *h_adj = nullptr;  // pinned copy of B.adj_p
 };
 
 namespace {
@@ -179,11 +181,13 @@ void free_bufs(nbx_tree *t)
                     B.first,   B.alloc,   B.bin_com, B.bin_cell, B.rec_a,  B.rec_b,
                     B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels,
                     B.step_ctr, B.dtab,    B.pflag2,  B.prefix2, B.pagg2,   B.pinc2,
-                    B.bin_q,   B.rec_q,   B.lo,      B.diag};
+                    B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
+                    B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
     if (t->h_dtab) (void)hipHostFree(t->h_dtab);
+    if (t->h_adj) (void)hipHostFree(t->h_adj);
     if (t->dtab_done) (void)hipEventDestroy(t->dtab_done);
     if (t->graph) (void)hipGraphExecDestroy(t->graph);
     B = Bufs{};
@@ -888,6 +892,162 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     }
     t->stage = kSummarized;
     return finish(t);
+}
+
+// ---------------------------------------------------------------------------
+// Reverse mode of the collision-free step (differentiable window)
+// ---------------------------------------------------------------------------
+
+int nbx_tree_set_adjoint(nbx_tree *t, int enable)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (!enable || t->adjoint) {
+        t->adjoint = t->adjoint && enable;
+        return NBX_OK;
+    }
+    Bufs &B = t->B;
+    const int n = t->T.n_total;
+    int rc;
+    if ((rc = dmalloc(&B.brange, n)) || (rc = dmalloc(&B.rec_r, B.n_rec)) ||
+        (rc = dmalloc(&B.adj_g, 3 * (size_t)B.n_rec)) ||
+        (rc = dmalloc(&B.adj_d, 3 * (size_t)(n + 1))) ||
+        (rc = dmalloc(&B.adj_blk, 3 * (size_t)adj_scan_blocks(n))) || (rc = dmalloc(&B.adj_p, 4)))
+        return rc;
+    HIPCHK(hipMemset(B.rec_r, 0, sizeof(int2) * B.n_rec));
+    HIPCHK(hipHostMalloc((void **)&t->h_adj, 4 * sizeof(double)));
+    if (t->graph) {  // the captured step holds the build without range outputs
+        HIPCHK(hipGraphExecDestroy(t->graph));
+        t->graph = nullptr;
+    }
+    t->adjoint = true;
+    return NBX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int adj_check(nbx_tree *t, const nbx_particles *p)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!t->adjoint) return fail(NBX_E_STATE, "reverse mode not enabled (nbx_tree_set_adjoint)");
+    if (t->T.nseg != 1) return fail(NBX_E_ARG, "reverse mode takes one disk per tree");
+    if (t->B.quad) return fail(NBX_E_ARG, "reverse mode differentiates the monopole walk only");
+    if (t->compensated) return fail(NBX_E_ARG, "reverse mode follows the float KDK only");
+    return NBX_OK;
+}
+
+// Perturber of the force evaluation at sim_time (Perturber.h:60-66 in float,
+// as the forward step) and dX_p/dr_p of the orbit r (cos phi, 0, sin phi),
+// phi = phase0 + omega t, omega = sqrt(M / r^3) (Perturber.h:39-58).
+AdjWalkArgs adj_args(const nbx_tree *t, float theta, float softening, float star_mass,
+                     const nbx_disk_params *dp, float sim_time, const double *ab, double *xb)
+{
+    AdjWalkArgs A{};
+    A.theta2 = theta * theta;
+    A.eps2 = softening * softening;
+    A.softening = softening;
+    A.star_mass = star_mass;
+    A.targets = t->walk_tg;
+    A.ab = ab;
+    A.xb = xb;
+    if (dp && dp->perturber_enabled) {
+        const float phi = dp->pert_phase0 + dp->pert_omega * sim_time;
+        A.px = dp->pert_radius_au * cosf(phi);
+        A.py = 0.f;
+        A.pz = dp->pert_radius_au * sinf(phi);
+        A.pmass = dp->pert_mass_msun;
+        const double r = dp->pert_radius_au, w = dp->pert_omega, ph = phi;
+        const double dphi = (double)sim_time * (-1.5 * w / r);
+        A.drx = (float)(cos(ph) - r * sin(ph) * dphi);
+        A.dry = 0.f;
+        A.drz = (float)(sin(ph) + r * cos(ph) * dphi);
+    }
+    return A;
+}
+
+// Adjoint force over the tree of p's positions; pgrad += dL/d(m_p, r_p).
+int adj_force(nbx_tree *t, const nbx_particles *p, const AdjWalkArgs &A, double *pgrad)
+{
+    launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
+    launch_root(t->T, t->B, t->stream);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_summarize(t->T, t->B, t->stream);
+    t->stage = kSummarized;
+    launch_adj_force(t->T, A, t->B, t->stream);
+    HIPCHK(hipMemcpyAsync(t->h_adj, t->B.adj_p, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                          t->stream));
+    const bool was_async = t->async;
+    t->async = false;
+    const int rc = finish(t);
+    t->async = was_async;
+    if (rc) return rc;
+    if (pgrad) {
+        pgrad[0] += t->h_adj[0];
+        pgrad[1] += t->h_adj[1];
+    }
+    return NBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbx_adj_force(nbx_tree *t, const nbx_particles *p, float theta, float softening,
+                  float star_mass, const nbx_disk_params *dp, float sim_time, const double *abar,
+                  double *xbar, double *pgrad)
+{
+    int rc = adj_check(t, p);
+    if (rc) return rc;
+    if (!abar || !xbar) return fail(NBX_E_ARG, "null adjoint arrays");
+    return adj_force(t, p, adj_args(t, theta, softening, star_mass, dp, sim_time, abar, xbar),
+                     pgrad);
+}
+
+int nbx_adj_init_forces(nbx_tree *t, const nbx_particles *s0, const nbx_step_params *sp,
+                        const nbx_disk_params *dp, const double *abar, double *xbar,
+                        double *pgrad)
+{
+    int rc = adj_check(t, s0);
+    if (rc) return rc;
+    if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
+    if (!abar || !xbar) return fail(NBX_E_ARG, "null adjoint arrays");
+    // initial forces use the perturber at t = 0 (main.cu:85-96)
+    return adj_force(t, s0, adj_args(t, sp->theta, sp->softening, sp->star_mass, dp, 0.0f, abar,
+                                     xbar),
+                     pgrad);
+}
+
+int nbx_adj_step(nbx_tree *t, const nbx_particles *s0, const nbx_particles *s1,
+                 const nbx_step_params *sp, const nbx_disk_params *dp, int32_t step,
+                 double *xbar, double *vbar, double *abar, double *pgrad)
+{
+    int rc = adj_check(t, s1);
+    if (rc) return rc;
+    if (!particles_ok(s0) || s0->n != s1->n) return fail(NBX_E_ARG, "invalid step-start state");
+    if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
+    if (!xbar || !vbar || !abar) return fail(NBX_E_ARG, "null adjoint arrays");
+    if (sp->collisions) return fail(NBX_E_ARG, "reverse mode needs collisions off");
+    if (step < 0) return fail(NBX_E_ARG, "negative step index");
+    const int n = s1->n;
+    const float dt = sp->dt, dth = sp->dt * 0.5f;
+    const int has_drag = sp->gas_drag_rate > 0.0f;
+    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
+    launch_adj_kick_drag(s1->x, s1->z, s0->vx, s0->vy, s0->vz, s0->ax, s0->ay, s0->az, s1->ax,
+                         s1->ay, s1->az, dth, has_drag, decay, sp->star_mass, xbar, vbar, abar,
+                         n, t->stream);
+    const float sim_time = (float)(step + 1) * dt;  // main.cu:110
+    rc = adj_force(t, s1, adj_args(t, sp->theta, sp->softening, sp->star_mass, dp, sim_time,
+                                   abar, xbar),
+                   pgrad);
+    if (rc) return rc;
+    launch_adj_drift_kick(xbar, vbar, abar, dt, dth, n, t->stream);
+    const bool was_async = t->async;
+    t->async = false;
+    rc = finish(t);
+    t->async = was_async;
+    return rc;
 }
 
 int nbx_tree_stats(nbx_tree *t, int32_t *n_nodes, int32_t *max_level)

@@ -58,6 +58,13 @@ struct Bufs {
     DiskDev *dtab = nullptr;     // [dtab_cap * kMaxSeg] per-step disk values
     int dtab_cap = 0;
     int *step_ctr = nullptr;     // [1] step index into dtab, advanced by k_kick_drag
+    // reverse mode (allocated on demand)
+    int2 *brange = nullptr;      // [n] sorted-position range of binary nodes
+    int2 *rec_r = nullptr;       // [n_rec] sorted-position range of walk records
+    double *adj_g = nullptr;     // [3 n_rec] per-record sums of J(d) abar over accepting targets
+    double *adj_d = nullptr;     // [3 (n + 1)] range-add differences over sorted positions
+    double *adj_blk = nullptr;   // [3 blocks] block totals of the difference scan
+    double *adj_p = nullptr;     // [4] perturber parameter sums (mass, radius)
 };
 
 constexpr int kHistInts = 6 * 2048;  // histograms of the widest setting (6 x 11-bit)
@@ -143,5 +150,26 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
                     const float *vz, float restitution, Bufs &B, hipStream_t st);
 // FoF labels (original order) with link length ll for every disk.
 void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t st);
+
+// ---- reverse mode (collision-free step, monopole walk) ---------------------
+struct AdjWalkArgs {
+    float theta2, eps2, softening, star_mass;
+    int targets;                  // targets per wavefront: 64, 32, 16 or 8
+    float px, py, pz, pmass;      // perturber of this force evaluation (pmass 0: none)
+    float drx, dry, drz;          // dX_p / dr_p
+    const double *ab;             // [3 n] adjoint of the acceleration (original order)
+    double *xb;                   // [3 n] adjoint of the positions, accumulated
+};
+// xb += (dA/dx)^T ab over the current tree (built with B.brange / B.rec_r);
+// B.adj_p[0..1] = dL/dm_p, dL/dr_p of this evaluation.
+void launch_adj_force(const SegTable &T, const AdjWalkArgs &A, Bufs &B, hipStream_t st);
+int adj_scan_blocks(int n);
+void launch_adj_kick_drag(const float *x1, const float *z1, const float *vx0, const float *vy0,
+                          const float *vz0, const float *ax0, const float *ay0, const float *az0,
+                          const float *ax1, const float *ay1, const float *az1, float dt_half,
+                          int has_drag, float decay, float star_mass, double *xb, double *vb,
+                          double *ab, int n, hipStream_t st);
+void launch_adj_drift_kick(const double *xb, double *vb, double *ab, float dt, float dt_half,
+                           int n, hipStream_t st);
 
 }  // namespace nbx

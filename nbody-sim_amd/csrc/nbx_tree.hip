@@ -873,7 +873,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     SegTable T, const uint64_t *__restrict__ keys, const DVec<4> *__restrict__ P,
     const DVec<6> *__restrict__ P2, const float4 *__restrict__ bodies,
     const Root *__restrict__ roots, int *left, int *right, int *plen, int *octf,
-    float4 *bin_com, float4 *bin_cell, float4 *bin_q)
+    float4 *bin_com, float4 *bin_cell, float4 *bin_q, int2 *brange)
 {
 #pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -898,6 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     } while (t > 1);
     const int gamma = i + sp * d + min(d, 0);
     const int lo = min(i, jj), hi = max(i, jj);
+    if (brange) brange[j] = make_int2(T.off[s] + lo, T.off[s] + hi);
     left[j] = lo == gamma ? ~(T.off[s] + gamma) : T.ioff[s] + gamma;
     right[j] = hi == gamma + 1 ? ~(T.off[s] + gamma + 1) : T.ioff[s] + gamma + 1;
     plen[j] = dnode;
@@ -1025,9 +1026,13 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     const int *__restrict__ octf, const int *__restrict__ cnt, const int *__restrict__ first,
     const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
     const float4 *__restrict__ bodies, const float4 *__restrict__ bin_q, float4 *rec_a,
-    int4 *rec_b, float4 *rec_c, float4 *rec_q)
+    int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    // sorted-position range [lo, hi] covered by a record (adjoint only)
+    auto put_r = [&](int code, int r) {
+        if (rec_r) rec_r[r] = code < 0 ? make_int2(~code, ~code) : brange[code];
+    };
     auto put_q = [&](int code, int r) {
         if (!rec_q) return;
         if (code < 0) {
@@ -1058,6 +1063,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_b[T.noff[j]] = b;
         rec_c[T.noff[j]] = c;
         put_q(~T.off[j], T.noff[j]);
+        put_r(~T.off[j], T.noff[j]);
     }
     if (j >= T.ioff[T.nseg]) return;
     if (cnt[j] == 0) return;
@@ -1070,6 +1076,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_b[T.noff[s]] = b;
         rec_c[T.noff[s]] = c;
         put_q(j, T.noff[s]);
+        put_r(j, T.noff[s]);
     }
     const int f0 = first[j];
     oct_children(left, right, octf, j, [&](int k, int code) {
@@ -1080,6 +1087,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         rec_b[f0 + k] = b;
         rec_c[f0 + k] = c;
         put_q(code, f0 + k);
+        put_r(code, f0 + k);
     });
 }
 
@@ -1221,7 +1229,7 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
     if (ni > 0)
         hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[B.cur], B.prefix,
                            B.quad ? B.prefix2 : nullptr, B.bodies, B.roots, B.left, B.right,
-                           B.plen, B.oct, B.bin_com, B.bin_cell, B.bin_q);
+                           B.plen, B.oct, B.bin_com, B.bin_cell, B.bin_q, B.brange);
 }
 
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
@@ -1233,7 +1241,7 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
                            B.first, B.alloc, B.maxlev, B.err);
     hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.left, B.right,
                        B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a,
-                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr);
+                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r);
 }
 
 }  // namespace nbx

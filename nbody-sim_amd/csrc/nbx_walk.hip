@@ -13,227 +13,9 @@
 // Collision and FoF (per lane): the search radius is far below the particle
 // spacing, so the per-body paths diverge immediately; each lane descends on
 // its own stack, pruning cells by the exact distance to the cell box.
-#include <type_traits>
-
-#include "nbx_device.hpp"
-#include "nbx_launch.hpp"
+#include "nbx_wave.hpp"
 
 namespace nbx {
-
-static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
-
-constexpr int kWalkWaves = 1;   // one wavefront per block: walks never synchronise
-constexpr int kWalkBlock = kWave * kWalkWaves;
-constexpr int kGStack = 256;    // gravity stack entries per wave (deepest measured: 75)
-constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
-
-struct StackEntry {
-    int first, count;
-    unsigned mlo, mhi;
-};
-
-struct WaveLds {
-    StackEntry st[kGStack];
-    float4 a[kWave];
-    int4 b[kWave];
-    unsigned long long m[kWave];
-};
-
-struct WaveLdsQ : WaveLds {
-    float4 q[2 * kWave];
-};
-
-// Record data handed to a visitor (quadrupole rows only with QUAD).
-struct Rec {
-    float4 a;
-    int4 b;
-    float4 q0, q1;
-};
-
-// Wave-level walk over TG targets.  Lane = r * TG + t serves target t and
-// record slot r (L = 64 / TG slots): each popped batch of up to 64 records is
-// evaluated L records at a time, lane (t, r) taking record k0 + r for target
-// t.  Stack masks carry one bit per target.  With TG = 64 (L = 1) every lane
-// owns one target and evaluates every record of the batch.
-// Visitor: visit(rec, leaf) on the lanes that own the record's bit; returns
-// true when the lane opens the (internal) record.
-template <int TG, bool QUAD, class Lds, class Visitor>
-__device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
-                                 const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
-                                 int *err, Visitor &vis, unsigned long long *popped = nullptr)
-{
-    constexpr int LS = kWave / TG;
-    constexpr uint64_t TMASK = TG == 64 ? ~0ull : ((1ull << TG) - 1ull);
-    const int lane = __lane_id();
-    const int tt = lane % TG, rr = lane / TG;
-    // every lane writes the same entry: per-lane program order covers the
-    // later reads, no cross-lane ordering is needed
-    L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
-    int sp = 1, sp_max = 1;
-    while (sp > 0) {
-        // take up to 64 records from the top blocks (from the end of a block)
-        int nrec = 0, my_rec = -1;
-        uint64_t my_mask = 0;
-        while (sp > 0 && nrec < kWave) {
-            const StackEntry e = L.st[sp - 1];
-            const int count = rfl(e.count);
-            const int take = min(count, kWave - nrec);
-            if (lane >= nrec && lane < nrec + take) {
-                my_rec = e.first + (count - take) + (lane - nrec);
-                my_mask = ((uint64_t)e.mhi << 32) | (uint64_t)e.mlo;
-            }
-            if (take == count) {
-                --sp;
-            } else {
-                L.st[sp - 1].count = count - take;
-            }
-            nrec += take;
-        }
-        if (popped && lane == 0) atomicAdd(popped, (unsigned long long)nrec);
-        // one vector load per lane, staged in LDS
-        if (my_rec >= 0) {
-            L.a[lane] = ra[my_rec];
-            L.b[lane] = rb[my_rec];
-            L.m[lane] = my_mask;
-            if constexpr (QUAD) {
-                L.q[2 * lane] = rq[2 * my_rec];
-                L.q[2 * lane + 1] = rq[2 * my_rec + 1];
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k0 = 0; k0 < nrec; k0 += LS) {
-            // slots past the batch read a valid staging entry and are masked off
-            const int k = min(k0 + rr, nrec - 1);
-            const bool mine = (k0 + rr < nrec) && ((L.m[k] >> tt) & 1ull);
-            Rec r;
-            r.a = L.a[k];
-            r.b = L.b[k];
-            if constexpr (QUAD) {
-                r.q0 = L.q[2 * k];
-                r.q1 = L.q[2 * k + 1];
-            }
-            const bool open = vis.visit_masked(r, mine);
-            const uint64_t om = ballot64(open);
-            if (om) {
-#pragma unroll
-                for (int q = 0; q < LS; ++q) {
-                    const uint64_t mq = (om >> (q * TG)) & TMASK;
-                    if (mq) {
-                        const int4 b = L.b[k0 + q];
-                        if (sp < kGStack) {
-                            L.st[sp] = StackEntry{b.z, b.w, (unsigned)mq, (unsigned)(mq >> 32)};
-                            ++sp;
-                        } else if (lane == 0) {
-                            atomicOr(err, kErrStack);
-                        }
-                    }
-                }
-                sp_max = max(sp_max, sp);
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    if (lane == 0) atomicMax(err + 2, sp_max);  // deepest stack (diagnostic)
-}
-
-// Sum of a per-lane value over the LS record slots of target tt.
-template <int TG>
-__device__ inline float slot_sum(float v)
-{
-#pragma unroll
-    for (int o = TG; o < kWave; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-template <int TG>
-__device__ inline int slot_sum_i(int v)
-{
-#pragma unroll
-    for (int o = TG; o < kWave; o <<= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// ---------------------------------------------------------------------------
-// Gravity: accept a record when it is a leaf or h^2 < theta^2 r^2 (the
-// half-width / COM-distance test of BHTree.cu:576-577); skip the target's
-// own leaf (BHTree.cu:580); monopole pull m d / (r^2+eps^2)^{3/2}.
-// ---------------------------------------------------------------------------
-template <bool QUAD>
-struct GravityVisitor {
-    float bx, by, bz, theta2, eps2;
-    int self;
-    float fx, fy, fz;
-    // Branch-free: every lane evaluates the record and selects the result, so
-    // the wave never splits its execution mask inside the record loop.
-    __device__ inline bool visit_masked(const Rec &r, bool mine)
-    {
-        const float4 &a = r.a;
-        const int4 &b = r.b;
-        const bool leaf = b.w == 0;
-        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool open = mine && !leaf && !(__int_as_float(b.y) < theta2 * r2);
-        const bool take = mine && !open && !(leaf && b.z == self);
-        const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
-        const float rinv2 = rinv * rinv;
-        const float rinv3 = rinv2 * rinv;
-        const float f = take ? a.w * rinv3 : 0.f;
-        fx += f * dx;
-        fy += f * dy;
-        fz += f * dz;
-        if constexpr (QUAD) {
-            // a += -Q R / r^5 + (5/2) (R.Q R) R / r^7 for R = COM - body
-            // (Q traceless; zero for leaves)
-            const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
-            const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
-            const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
-            const float rinv5 = take ? rinv3 * rinv2 : 0.f;
-            const float g = 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2;
-            fx += g * dx - rinv5 * qx;
-            fy += g * dy - rinv5 * qy;
-            fz += g * dz - rinv5 * qz;
-        }
-        return open;
-    }
-};
-
-// Interaction counts of the same walk (accepted records excluding the own
-// leaf, opened internal records): the algorithmic work of the force pass.
-struct CountVisitor {
-    float bx, by, bz, theta2;
-    int self;
-    int n_acc, n_open;
-    __device__ inline bool visit_masked(const Rec &r, bool mine)
-    {
-        return mine && visit(r, r.b.w == 0);
-    }
-    __device__ inline bool visit(const Rec &r, bool leaf)
-    {
-        const float4 &a = r.a;
-        const int4 &b = r.b;
-        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
-        if (open) ++n_open;
-        else if (!(leaf && b.z == self)) ++n_acc;
-        return open;
-    }
-};
-
-// Waves per disk for a given number of targets per wave.
-struct WaveMap {
-    int wave[kMaxSeg + 1];
-};
-
-__device__ inline int seg_of_wmap(const SegTable &T, const WaveMap &M, int w)
-{
-    int s = 0;
-    while (s + 1 < T.nseg && w >= M.wave[s + 1]) ++s;
-    return s;
-}
 
 template <int TG, bool QUAD>
 __global__ __launch_bounds__(kWalkBlock) void k_walk(

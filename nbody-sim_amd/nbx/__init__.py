@@ -25,7 +25,7 @@ _PKG = os.path.dirname(_HERE)
 _REPO = os.path.dirname(_PKG)
 CSRC = os.path.join(_PKG, "csrc")
 LIB_PATH = os.path.join(_HERE, "libnbx.so")
-HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_abi.hip"]
+HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_adjoint.hip", "nbx_abi.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep hand-written float code on single-lane VALU ops (no v_pk_*_f32)
@@ -80,7 +80,8 @@ EXPORTS = [
     "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_walk_counts",
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
     "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
-    "nbx_tree_set_radix_bits",
+    "nbx_tree_set_radix_bits", "nbx_tree_set_adjoint", "nbx_adj_force", "nbx_adj_step",
+    "nbx_adj_init_forces",
 ]
 
 _lib = None
@@ -123,6 +124,13 @@ def lib():
             "nbx_tune_walk": [vp, f32, f32, f32, C.POINTER(f32)],
             "nbx_walk_stats": [vp, f32, C.POINTER(C.c_double)],
             "nbx_tree_set_radix_bits": [vp, i32],
+            "nbx_tree_set_adjoint": [vp, i32],
+            "nbx_adj_force": [vp, P, f32, f32, f32, C.POINTER(nbx_disk_params), f32, vp, vp,
+                              C.POINTER(C.c_double)],
+            "nbx_adj_step": [vp, P, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params),
+                             i32, vp, vp, vp, C.POINTER(C.c_double)],
+            "nbx_adj_init_forces": [vp, P, C.POINTER(nbx_step_params),
+                                    C.POINTER(nbx_disk_params), vp, vp, C.POINTER(C.c_double)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)

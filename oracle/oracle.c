@@ -722,6 +722,7 @@ void orc_mt_sort(const uint64_t *keys, int n, int32_t *perm)
 typedef struct {
     float cx, cy, cz, m, h2;
     int first, count; /* children are mt_child[first .. first+count) */
+    int lo, hi;       /* sorted positions covered: [lo, hi] */
     double q[6];      /* traceless quadrupole about the COM: xx yy zz xy xz yz */
 } mt_node;
 
@@ -813,6 +814,8 @@ static int mt_build_rec(mt_tree *T, int a, int b)
     }
     float h = ldexpf(T->rhs, -level);
     nd->h2 = h * h;
+    nd->lo = a;
+    nd->hi = b - 1;
     nd->first = first;
     nd->count = nruns;
     return id;
@@ -929,6 +932,98 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
     }
     free(keys); free(skeys); free(perm); free(sb); free(T.nodes); free(T.child);
     return overflow;
+}
+
+/* Interaction lists of the monopole walk of orc_mt_force: for every target
+ * (original order) the accepted nodes as sorted-position ranges [lo, hi]
+ * (a leaf has lo == hi; the target's own leaf is skipped).  offsets[n + 1]
+ * is filled always; lo / hi (capacity `cap`) only when both are non-NULL.
+ * perm[s] = original index of sorted position s.  Returns the total number
+ * of accepted nodes, or -1 when cap is too small. */
+long orc_mt_lists(const float *x, const float *y, const float *z, const float *m, int n,
+                  float theta, int32_t *perm, int64_t *offsets, int32_t *lo, int32_t *hi,
+                  long cap)
+{
+    if (n <= 0) return 0;
+    float root[4];
+    orc_mt_root(x, y, z, n, root);
+    uint64_t *keys = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    uint64_t *skeys = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    float *sb = (float *)malloc((size_t)n * 4 * sizeof(float));
+    orc_mt_keys(x, y, z, n, root, keys);
+    orc_mt_sort(keys, n, perm);
+    for (int s = 0; s < n; ++s) {
+        skeys[s] = keys[perm[s]];
+        sb[s] = x[perm[s]];
+        sb[n + s] = y[perm[s]];
+        sb[2 * n + s] = z[perm[s]];
+        sb[3 * n + s] = m[perm[s]];
+    }
+    mt_tree T;
+    memset(&T, 0, sizeof(T));
+    T.keys = skeys;
+    T.sx = sb; T.sy = sb + n; T.sz = sb + 2 * n; T.sm = sb + 3 * n;
+    T.nodes = (mt_node *)malloc((size_t)(n > 1 ? n : 1) * sizeof(mt_node));
+    T.child = (int *)malloc((size_t)2 * (n > 1 ? n : 1) * sizeof(int));
+    T.rhs = root[3];
+    if (n > 1) mt_build_rec(&T, 0, n);
+    const float theta2 = theta * theta;
+    int32_t *cnt = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+    const int fill = lo && hi;
+    long total = 0;
+    for (int pass = 0; pass < (fill ? 2 : 1); ++pass) {
+        if (pass == 1) {
+            offsets[0] = 0;
+            for (int i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+            total = offsets[n];
+            if (total > cap) { total = -1; break; }
+        }
+        for (int s = 0; s < n && n > 1; ++s) {
+            const int i = perm[s];
+            long w = pass == 1 ? offsets[i] : 0;
+            int c_i = 0;
+            const float bx = T.sx[s], by = T.sy[s], bz = T.sz[s];
+            int stack[512];
+            int top = 0;
+            const mt_node *r = &T.nodes[0];
+            float dx = r->cx - bx, dy = r->cy - by, dz = r->cz - bz;
+            if (r->h2 < theta2 * (dx * dx + dy * dy + dz * dz)) {
+                if (pass == 1) { lo[w] = r->lo; hi[w] = r->hi; ++w; }
+                ++c_i;
+            } else {
+                stack[top++] = 0;
+            }
+            while (top > 0) {
+                const mt_node *p = &T.nodes[stack[--top]];
+                for (int k = 0; k < p->count; ++k) {
+                    const int c = T.child[p->first + k];
+                    if (c < 0) {
+                        const int b = ~c;
+                        if (b == s) continue;
+                        if (pass == 1) { lo[w] = b; hi[w] = b; ++w; }
+                        ++c_i;
+                        continue;
+                    }
+                    const mt_node *nd = &T.nodes[c];
+                    dx = nd->cx - bx; dy = nd->cy - by; dz = nd->cz - bz;
+                    if (nd->h2 < theta2 * (dx * dx + dy * dy + dz * dz)) {
+                        if (pass == 1) { lo[w] = nd->lo; hi[w] = nd->hi; ++w; }
+                        ++c_i;
+                    } else if (top < 512) {
+                        stack[top++] = c;
+                    }
+                }
+            }
+            cnt[i] = c_i;
+        }
+        if (pass == 0 && !fill) {
+            offsets[0] = 0;
+            for (int i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+            total = offsets[n];
+        }
+    }
+    free(cnt); free(keys); free(skeys); free(sb); free(T.nodes); free(T.child);
+    return total;
 }
 
 /* ======================================================================== */

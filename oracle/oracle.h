@@ -161,6 +161,14 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
                  float *self_ax, float *self_ay, float *self_az,
                  int32_t *n_acc, int32_t *n_open, int nthreads);
 
+/* Interaction lists of the same monopole walk: per target (original order)
+ * the accepted nodes as sorted-position ranges [lo, hi] (leaf: lo == hi).
+ * offsets[n+1] always; lo/hi (capacity cap) when both non-NULL; perm[n] =
+ * sorted -> original.  Returns the total, or -1 if cap is too small. */
+long orc_mt_lists(const float *x, const float *y, const float *z, const float *m, int n,
+                  float theta, int32_t *perm, int64_t *offsets, int32_t *lo, int32_t *hi,
+                  long cap);
+
 /* ------------------------------------------------------------------------ */
 /* Whole reference step (main.cu:101-140), for the CPU baseline               */
 /* ------------------------------------------------------------------------ */

@@ -100,6 +100,9 @@ def lib():
         L.orc_mt_force.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_float, C.c_float,
                                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int]
+        L.orc_mt_lists.restype = C.c_long
+        L.orc_mt_lists.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_float, _i32p,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
         L.orc_ref_init_forces.argtypes = [_f32p] * 10 + [C.c_int, C.POINTER(StepParams), C.c_int]
         L.orc_ref_steps.argtypes = [_f32p] * 10 + [C.c_int, C.POINTER(StepParams), C.c_int,
                                                   C.c_int, C.c_int]
@@ -276,6 +279,22 @@ def mt_force(x, y, z, m, theta, softening, counts=False, quad=False, nthreads=0)
     if counts:
         out["n_acc"], out["n_open"] = na, no
     return out
+
+
+def mt_lists(x, y, z, m, theta):
+    """Interaction lists of mt_force's monopole walk: perm (sorted -> original),
+    CSR offsets per target (original order) and the accepted nodes' sorted
+    ranges lo, hi (inclusive)."""
+    x, y, z, m = _f32(x), _f32(y), _f32(z), _f32(m)
+    n = len(x)
+    perm = np.zeros(n, np.int32)
+    off = np.zeros(n + 1, np.int64)
+    total = lib().orc_mt_lists(x, y, z, m, n, theta, perm, _ptr(off), None, None, 0)
+    lo = np.zeros(max(total, 1), np.int32)
+    hi = np.zeros(max(total, 1), np.int32)
+    got = lib().orc_mt_lists(x, y, z, m, n, theta, perm, _ptr(off), _ptr(lo), _ptr(hi), total)
+    assert got == total
+    return perm, off, lo[:total], hi[:total]
 
 
 def step_params(**kw):
