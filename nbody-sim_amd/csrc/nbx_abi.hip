@@ -67,6 +67,9 @@ struct nbx_tree {
     bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
     int walk_tg = kDefaultTargets;  // gravity-walk targets per wavefront
     float tune_ms[4] = {0, 0, 0, 0};  // autotuner timings for 64, 32, 16, 8 targets
+    // side stream of the collision pass, which overlaps the force walk
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
     double *h_adj = nullptr;  // pinned copy of B.adj_p
 };
@@ -134,10 +137,12 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
     ALLOC(B.tickets, 32);
     ALLOC(B.prefix, n + t->T.nseg);
-    ALLOC(B.pflag, B.n_scan_tiles);
-    ALLOC(B.pagg, B.n_scan_tiles);
-    ALLOC(B.pinc, B.n_scan_tiles);
-    ALLOC(B.pflag2, B.n_scan_tiles);
+    // scan look-back words of both prefix scans: 4 + 4 + 6 + 6 doubles per tile
+    ALLOC(B.scanw, (size_t)kScanWords * B.n_scan_tiles);
+    B.pagg = (DVec<4> *)B.scanw;
+    B.pinc = B.pagg + B.n_scan_tiles;
+    B.pagg2 = (DVec<6> *)(B.scanw + 8 * (size_t)B.n_scan_tiles);
+    B.pinc2 = B.pagg2 + B.n_scan_tiles;
     ALLOC(B.bodies, n);
     ALLOC(B.left, n);
     ALLOC(B.right, n);
@@ -168,6 +173,7 @@ int alloc_bufs(nbx_tree *t)
     HIPCHK(hipMemset(B.ghist, 0, kHistInts * sizeof(int)));
     HIPCHK(hipMemset(B.tickets, 0, 32 * sizeof(int)));
     HIPCHK(hipMemset(B.alloc, 0, 2 * kMaxSeg * sizeof(int)));
+
     HIPCHK(hipHostMalloc((void **)&t->h_err, 4 * sizeof(int)));
     return NBX_OK;
 }
@@ -176,11 +182,11 @@ void free_bufs(nbx_tree *t)
 {
     Bufs &B = t->B;
     void *ptrs[] = {B.keys[0], B.keys[1], B.vals[0], B.vals[1], B.ghist,   B.status[0],
-                    B.status[1], B.tickets, B.prefix, B.pflag,  B.pagg,    B.pinc,
+                    B.status[1], B.tickets, B.prefix, B.scanw,
                     B.bodies,  B.left,    B.right,   B.plen,    B.oct,     B.cnt,
                     B.first,   B.alloc,   B.bin_com, B.bin_cell, B.rec_a,  B.rec_b,
                     B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels,
-                    B.step_ctr, B.dtab,    B.pflag2,  B.prefix2, B.pagg2,   B.pinc2,
+                    B.step_ctr, B.dtab,    B.prefix2,
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p};
     for (void *p : ptrs)
@@ -253,6 +259,16 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
         return fail(NBX_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     t->own_stream = true;
+    if ((e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming)) != hipSuccess) {
+        std::string msg = std::string("stream / event setup: ") + hipGetErrorString(e);
+        if (t->side) (void)hipStreamDestroy(t->side);
+        if (t->fork) (void)hipEventDestroy(t->fork);
+        (void)hipStreamDestroy(t->stream);
+        delete t;
+        return fail(NBX_E_HIP, "%s", msg.c_str());
+    }
     int rc = alloc_bufs(t);
     if (rc != NBX_OK) {
         std::string msg = g_err;
@@ -402,8 +418,12 @@ int nbx_tree_destroy(nbx_tree *t)
     if (!t) return NBX_OK;
     (void)hipStreamSynchronize(t->stream);
     for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+    (void)hipStreamSynchronize(t->side);
     free_bufs(t);
     if (t->own_stream) (void)hipStreamDestroy(t->stream);
+    (void)hipStreamDestroy(t->side);
+    (void)hipEventDestroy(t->fork);
+    (void)hipEventDestroy(t->join);
     delete t;
     return NBX_OK;
 }
@@ -439,8 +459,6 @@ int nbx_tree_set_multipole(nbx_tree *t, int order)
         const int n = t->T.n_total;
         int rc;
         if ((rc = dmalloc(&B.prefix2, (size_t)n + t->T.nseg)) != NBX_OK) return rc;
-        if ((rc = dmalloc(&B.pagg2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
-        if ((rc = dmalloc(&B.pinc2, (size_t)B.n_scan_tiles)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.bin_q, (size_t)2 * n)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.rec_q, (size_t)2 * B.n_rec)) != NBX_OK) return rc;
         HIPCHK(hipMemset(B.rec_q, 0, sizeof(float4) * 2 * B.n_rec));
@@ -801,9 +819,21 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);
+    // The collision pass reads the tree and the velocities and writes only
+    // B.dv; the walk writes only the accelerations: without phase timing the
+    // collision pass runs on the side stream, filling the walk's idle slots.
+    const bool overlap = sp->collisions && !t->timing;
+    if (overlap) (void)hipEventRecord(t->fork, t->stream);
     launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
+    if (overlap) {  // submitted after the walk, so the walk's waves go first
+        (void)hipStreamWaitEvent(t->side, t->fork, 0);
+        launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->side);
+        (void)hipEventRecord(t->join, t->side);
+    }
     if (e0 >= 0) ev_record(t, e0 + 3);
-    if (sp->collisions)
+    if (overlap)
+        (void)hipStreamWaitEvent(t->stream, t->join, 0);
+    else if (sp->collisions)
         launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 4);
     if (t->compensated)

@@ -24,12 +24,13 @@ struct Bufs {
     unsigned *status[2] = {};    // [n_sweep_tiles * 2048] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
     DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
-    int *pflag = nullptr;        // [n_scan_tiles] scan look-back flags
+    // scan look-back words (one allocation, reset to kScanEmpty by the key
+    // pass): tile aggregates and inclusive tile prefixes of both scans
+    double *scanw = nullptr;     // [kScanWords * n_scan_tiles]
     DVec<4> *pagg = nullptr;     // [n_scan_tiles] tile aggregates
     DVec<4> *pinc = nullptr;     // [n_scan_tiles] inclusive tile prefixes
     int quad = 0;                // 1: quadrupole moments are built and used
     DVec<6> *prefix2 = nullptr;  // [n + 1] exclusive prefix of the second moments
-    int *pflag2 = nullptr;
     DVec<6> *pagg2 = nullptr, *pinc2 = nullptr;
     float4 *bin_q = nullptr;     // [2 n] traceless quadrupole of binary nodes
     float4 *rec_q = nullptr;     // [2 n_rec] quadrupole of walk records (0 for leaves)
@@ -47,7 +48,7 @@ struct Bufs {
     int4 *rec_b = nullptr;       // walk records {h, h^2, first, count}
     float4 *rec_c = nullptr;     // walk records {cell centre | body, h}
     int n_rec = 0;
-    float *partial = nullptr;    // [blocks * 6] bbox partials
+    float *partial = nullptr;    // [blocks * 6] per-block bbox min / max
     Root *roots = nullptr;       // [kMaxSeg]
     float *dv = nullptr;         // [3 n] collision velocity deltas
     int *err = nullptr;          // [4] error flags / counters
@@ -67,6 +68,9 @@ struct Bufs {
     double *adj_p = nullptr;     // [4] perturber parameter sums (mass, radius)
 };
 
+constexpr int kScanWords = 4 + 4 + 6 + 6;
+// bit pattern of a look-back word not yet written (a NaN no sum produces)
+constexpr unsigned long long kScanEmpty = 0x7ff8dead0badf00dull;
 constexpr int kHistInts = 6 * 2048;  // histograms of the widest setting (6 x 11-bit)
 constexpr int kSweepItems = 8;
 constexpr int kSweepTile = kBlock * kSweepItems;
@@ -76,7 +80,6 @@ constexpr int kScanTile = kBlock * kScanItems;
 constexpr int kTicketPass = 0;   // + pass (9 slots incl. the disk pass)
 constexpr int kTicketScan = 10;
 constexpr int kTicketScan2 = 12;
-constexpr int kBlockCtr = 15;    // last-block counter of the bbox reduction
 
 
 // ---- integrator (reference-identical float ops) --------------------------
@@ -92,7 +95,7 @@ void launch_perturber(float *ax, float *ay, float *az, const float *x, const flo
                       int n, hipStream_t st);
 
 // ---- fused step kernels ---------------------------------------------------
-// v += a*dt_half; x += v*dt; then per-block bbox partials of the new x.
+// v += a*dt_half; x += v*dt; then the per-block boxes of the new x.
 void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, float *vx,
                             float *vy, float *vz, const float *ax, const float *ay,
                             const float *az, float dt_half, float dt, Bufs &B, hipStream_t st);

@@ -1,7 +1,7 @@
 // nbx_tree.hip -- octree construction and integrator kernels for gfx950.
 //
 // Build pipeline (one launch per phase for all disks of a batch):
-//   bbox partials + root cube (last block) -> 63-bit keys + digit histograms
+//   per-block boxes -> root cube + 63-bit keys + digit histograms
 //   -> 8 stable radix passes with decoupled look-back (the last one gathers
 //   the sorted bodies) -> FP64 prefix scan of {m, m x, m y, m z} -> binary
 //   radix tree (one thread per internal node) with node mass / COM from the
@@ -80,15 +80,15 @@ __global__ void k_perturber(float *ax, float *ay, float *az, const float *__rest
     az[i] = c;
 }
 
-// Fused first half-kick + drift (main.cu:104-107) with the bbox partial of
-// the drifted positions (BHTree.cu:94-133) in the same pass over HBM.
-__device__ void bbox_finish(const SegTable &T, float lx, float ly, float lz, float hx, float hy,
-                            float hz, float *partial, Root *roots, int *tickets, int *ghist);
+// Fused first half-kick + drift (main.cu:104-107) with the bbox of the
+// drifted positions (BHTree.cu:94-133) in the same pass over HBM.
+__device__ void bbox_post(float lx, float ly, float lz, float hx, float hy, float hz,
+                          float *partial);
 
 __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox(
     SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
     const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
-    float dt_half, float dt, float *partial, Root *roots, int *tickets, int *ghist)
+    float dt_half, float dt, float *partial)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox(
         ly = hy = yi;
         lz = hz = zi;
     }
-    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, ghist);
+    bbox_post(lx, ly, lz, hx, hy, hz, partial);
 }
 
 // Collision delta, second half-kick and gas drag (main.cu:131-139 after the
@@ -149,7 +149,7 @@ __global__ void k_kick_drag(float *vx, float *vy, float *vz, const float *__rest
 __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox_c(
     SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
     const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
-    float dt_half, float dt, float *lo, float *partial, Root *roots, int *tickets)
+    float dt_half, float dt, float *lo, float *partial)
 {
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox_c(
         ly = hy = o[1];
         lz = hz = o[2];
     }
-    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, nullptr);
+    bbox_post(lx, ly, lz, hx, hy, hz, partial);
 }
 
 __device__ inline void gas_drag_one_d(double xi, double zi, double &vxi, double &vyi,
@@ -342,10 +342,9 @@ __global__ void k_fill_float(float *p, float v, int n)
 }
 
 // ===========================================================================
-// Bounding box and root cube.  Every block writes its min/max partial; the
-// last block to finish reduces the partials of each disk, pads the box by
-// 1.001 about its midpoint (BHTree.cu:440-446), forms the root cube
-// (BHTree.cu:187-191) and clears the counters of the build that follows.
+// Bounding box and root cube.  Every block stores its min/max; the key pass
+// reduces them per disk, pads the box by 1.001 about its midpoint
+// (BHTree.cu:440-446) and forms the root cube (BHTree.cu:187-191).
 // ===========================================================================
 
 __device__ void root_from_box(const float *lo_hi, Root *out)
@@ -367,44 +366,42 @@ __device__ void root_from_box(const float *lo_hi, Root *out)
     *out = R;
 }
 
-__device__ void bbox_finish(const SegTable &T, float lx, float ly, float lz, float hx, float hy,
-                            float hz, float *partial, Root *roots, int *tickets, int *)
+// Each block stores its min / max; the key pass, the next kernel, reduces
+// the partials of its disks (no atomics, no fence: the kernel boundary
+// orders the stores before the loads).
+__device__ void bbox_post(float lx, float ly, float lz, float hx, float hy, float hz,
+                          float *partial)
 {
-    __shared__ int s_last;
-    __shared__ float s_box[6];
     block_minmax6(lx, ly, lz, hx, hy, hz, partial + 6 * blockIdx.x);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int prev = atomicAdd(&tickets[kBlockCtr], 1);
-        s_last = prev == (int)gridDim.x - 1;
+}
+
+// Root cube of disk s from the block partials (BHTree.cu:440-446, :187-191).
+// Called by every thread of the block; the result is valid in *out after
+// the trailing barrier.
+__device__ void root_of_partials(const SegTable &T, const float *__restrict__ partial, int s,
+                                 Root *out)
+{
+    __shared__ float s_box[6];
+    float a = FLT_MAX, b = FLT_MAX, c = FLT_MAX, d = -FLT_MAX, e = -FLT_MAX, f = -FLT_MAX;
+    for (int q = T.blk[s] + threadIdx.x; q < T.blk[s + 1]; q += kBlock) {
+        const float *p = partial + 6 * q;
+        a = fminf(a, p[0]);
+        b = fminf(b, p[1]);
+        c = fminf(c, p[2]);
+        d = fmaxf(d, p[3]);
+        e = fmaxf(e, p[4]);
+        f = fmaxf(f, p[5]);
     }
+    block_minmax6(a, b, c, d, e, f, s_box);
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    for (int s = 0; s < T.nseg; ++s) {
-        float a = FLT_MAX, b = FLT_MAX, c = FLT_MAX, d = -FLT_MAX, e = -FLT_MAX, f = -FLT_MAX;
-        for (int q = T.blk[s] + threadIdx.x; q < T.blk[s + 1]; q += kBlock) {
-            const float *p = partial + 6 * q;
-            a = fminf(a, __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            b = fminf(b, __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            c = fminf(c, __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            d = fmaxf(d, __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            e = fmaxf(e, __hip_atomic_load(p + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            f = fmaxf(f, __hip_atomic_load(p + 5, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        }
-        block_minmax6(a, b, c, d, e, f, s_box);
-        __syncthreads();
-        if (threadIdx.x == 0) root_from_box(s_box, &roots[s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) tickets[kBlockCtr] = 0;
+    if (threadIdx.x == 0) root_from_box(s_box, out);
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__restrict__ x,
                                                  const float *__restrict__ y,
-                                                 const float *__restrict__ z, float *partial,
-                                                 Root *roots, int *tickets, int *ghist)
+                                                 const float *__restrict__ z,
+                                                 float *partial)
 {
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
@@ -416,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__rest
         ly = hy = y[i];
         lz = hz = z[i];
     }
-    bbox_finish(T, lx, ly, lz, hx, hy, hz, partial, roots, tickets, ghist);
+    bbox_post(lx, ly, lz, hx, hy, hz, partial);
 }
 
 // ===========================================================================
@@ -434,24 +431,32 @@ template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__restrict__ x,
                                                  const float *__restrict__ y,
                                                  const float *__restrict__ z,
-                                                 const Root *__restrict__ roots,
+                                                 const float *__restrict__ partial,
+                                                 Root *__restrict__ roots,
                                                  uint64_t *__restrict__ keys,
                                                  int *__restrict__ vals, int *__restrict__ ghist,
                                                  unsigned *__restrict__ status0, int n_status,
-                                                 int *__restrict__ pflag, int n_pflag,
-                                                 int *__restrict__ pflag2,
-                                                 int *__restrict__ tickets)
+                                                 unsigned long long *__restrict__ scanw,
+                                                 int n_scanw, int *__restrict__ tickets)
 {
 #pragma clang fp contract(off)
     constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
     __shared__ int h[NP * R];
+    __shared__ Root s_root[kMaxSeg];
     for (int q = threadIdx.x; q < NP * R; q += kBlock) h[q] = 0;
+    {
+        // root cubes of the disks this block's bodies belong to (block 0:
+        // of every disk, published for the later passes)
+        const int b0 = blockIdx.x * kKeysPerBlock;
+        const int b1 = min(T.n_total, b0 + kKeysPerBlock) - 1;
+        const int s0 = blockIdx.x == 0 ? 0 : seg_of_body(T, b0);
+        const int s1 = blockIdx.x == 0 ? T.nseg - 1 : seg_of_body(T, b1);
+        for (int sg = s0; sg <= s1; ++sg) root_of_partials(T, partial, sg, &s_root[sg]);
+        if (blockIdx.x == 0 && threadIdx.x < T.nseg) roots[threadIdx.x] = s_root[threadIdx.x];
+    }
     const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
-    for (int q = gid; q < n_pflag; q += gsz) {
-        pflag[q] = 0;
-        pflag2[q] = 0;
-    }
+    for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
     if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
     __syncthreads();
     const int n = T.n_total;
@@ -467,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
     for (int r = 0; r < kKeysItems; ++r) {
         const int i = blockIdx.x * kKeysPerBlock + r * kBlock + threadIdx.x;
         if (i >= n) break;
-        const Root R0 = roots[seg_of_body(T, i)];
+        const Root R0 = s_root[seg_of_body(T, i)];
         float cx = R0.c[0], cy = R0.c[1], cz = R0.c[2], hh = R0.c[3];
         uint64_t key = 0;
 #pragma unroll
@@ -705,6 +710,31 @@ __device__ inline DVec<N> dv_add(const DVec<N> &a, const DVec<N> &b)
     return r;
 }
 
+// Look-back words: one relaxed 64-bit atomic per double (no fence needed:
+// a reader only uses a value whose N words all differ from kScanEmpty).
+template <int N>
+__device__ inline void dv_publish(DVec<N> *p, const DVec<N> &a)
+{
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+        __hip_atomic_store((unsigned long long *)&p->v[q],
+                           (unsigned long long)__double_as_longlong(a.v[q]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int N>
+__device__ inline bool dv_fetch(const DVec<N> *p, DVec<N> &a)
+{
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const unsigned long long u = __hip_atomic_load((unsigned long long *)&p->v[q],
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = ok && u != kScanEmpty;
+        a.v[q] = __longlong_as_double((long long)u);
+    }
+    return ok;
+}
+
 template <int N>
 __device__ inline DVec<N> dv_zero()
 {
@@ -730,7 +760,7 @@ __host__ __device__ inline int pbase(const SegTable &T, int s) { return T.off[s]
 
 template <int N, int KIND>
 __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__restrict__ bodies,
-                                                   DVec<N> *P, int *pflag, DVec<N> *pagg,
+                                                   DVec<N> *P, DVec<N> *pagg,
                                                    DVec<N> *pinc, int *tickets, int slot,
                                                    int *ghist)
 {
@@ -789,36 +819,37 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
     }
     if (w == 0) {
         // publish, then look back over up to 64 earlier tiles of this disk at
-        // once: add aggregates up to the nearest inclusive prefix
+        // once: add aggregates up to the nearest inclusive prefix.  Every
+        // word is stored and loaded as one 64-bit atomic; a tile's value is
+        // complete when none of its N words is still kScanEmpty.
         if (tile == first_tile) {
             if (lane == 0) {
-                pinc[tile] = agg;
-                __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                dv_publish(pinc + tile, agg);
+                s_pre = dv_zero<N>();
             }
-            if (lane == 0) s_pre = dv_zero<N>();
         } else {
-            if (lane == 0) {
-                pagg[tile] = agg;
-                __hip_atomic_store(&pflag[tile], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (lane == 0) dv_publish(pagg + tile, agg);
             DVec<N> pre = dv_zero<N>();
             int j0 = tile - 1;
             while (true) {
                 const int j = j0 - lane;
-                int f = 2;  // before the disk's first tile: nothing to add
+                DVec<N> ci = dv_zero<N>(), ca = dv_zero<N>();
+                bool inc = true;  // before the disk's first tile: nothing to add
                 if (j >= first_tile) {
-                    f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    while (f == 0) {
+                    while (true) {
+                        if (dv_fetch(pinc + j, ci)) break;
+                        if (dv_fetch(pagg + j, ca)) {
+                            inc = false;
+                            break;
+                        }
                         __builtin_amdgcn_s_sleep(1);
-                        f = __hip_atomic_load(&pflag[j], __ATOMIC_ACQUIRE,
-                                              __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
-                const uint64_t incm = ballot64(f == 2);
+                const uint64_t incm = ballot64(inc);
                 const int stop = incm ? __ffsll((long long)incm) - 1 : 64;  // nearest inclusive
                 DVec<N> c = dv_zero<N>();
-                if (lane < stop) c = pagg[j];
-                else if (lane == stop && j >= first_tile) c = pinc[j];
+                if (lane < stop) c = ca;
+                else if (lane == stop) c = ci;
 #pragma unroll
                 for (int q = 0; q < N; ++q)
 #pragma unroll
@@ -828,8 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
                 j0 -= 64;
             }
             if (lane == 0) {
-                pinc[tile] = dv_add(pre, agg);
-                __hip_atomic_store(&pflag[tile], 2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                dv_publish(pinc + tile, dv_add(pre, agg));
                 s_pre = pre;
             }
         }
@@ -1137,8 +1167,7 @@ void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, flo
                             const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
 {
     hipLaunchKernelGGL(k_kick_drift_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial, B.roots, B.tickets,
-                       B.ghist);
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial);
 }
 
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -1155,7 +1184,7 @@ void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, f
                               const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
 {
     hipLaunchKernelGGL(k_kick_drift_bbox_c, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                       vx, vy, vz, ax, ay, az, dt_half, dt, B.lo, B.partial, B.roots, B.tickets);
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.lo, B.partial);
 }
 
 void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -1181,8 +1210,8 @@ void launch_diagnostics(const float *x, const float *y, const float *z, const fl
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z, B.partial,
-                       B.roots, B.tickets, B.ghist);
+    hipLaunchKernelGGL(k_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
+                       B.partial);
 }
 
 void launch_root(const SegTable &, Bufs &, hipStream_t) {}
@@ -1194,8 +1223,9 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
     constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
     const int n = T.n_total;
     hipLaunchKernelGGL(k_keys<BITS>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
-                       z, B.roots, B.keys[0], B.vals[0], B.ghist, B.status[0],
-                       R * B.n_sweep_tiles, B.pflag, B.n_scan_tiles, B.pflag2, B.tickets);
+                       z, B.partial, B.roots, B.keys[0], B.vals[0], B.ghist, B.status[0],
+                       R * B.n_sweep_tiles, (unsigned long long *)B.scanw,
+                       kScanWords * B.n_scan_tiles, B.tickets);
     const int npass = NP + (T.nseg > 1 ? 1 : 0);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
@@ -1220,10 +1250,10 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
     default: launch_sort<8>(T, x, y, z, mass, B, st); break;
     }
     hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T, B.bodies,
-                       B.prefix, B.pflag, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist);
+                       B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist);
     if (B.quad)
         hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
-                           B.bodies, B.prefix2, B.pflag2, B.pagg2, B.pinc2, B.tickets,
+                           B.bodies, B.prefix2, B.pagg2, B.pinc2, B.tickets,
                            kTicketScan2, B.ghist);
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
