@@ -120,6 +120,11 @@ int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n
  * tree's current shape, host out[4] = {mean accepted per body, mean opened
  * per body, mean records evaluated per wavefront, deepest wave stack}. */
 int nbx_walk_stats(nbx_tree *t, float theta, double *out);
+/* No reference counterpart (diagnostic): per-wavefront work of the same walk,
+ * host out[3 * wave] = {records evaluated, record batches, deepest stack}.
+ * *n_waves receives the wavefront count; out may be NULL to query it, else
+ * it must hold cap >= *n_waves wavefronts. */
+int nbx_walk_wave_stats(nbx_tree *t, float theta, int32_t *out, int32_t cap, int32_t *n_waves);
 /* launchCollisionKernel (BHTree.cu:688-708): exact pair set
  * dist^2 < r_coll^2, impulse J = -(1+e) dvn / 2 for approaching pairs,
  * deltas accumulated per body in a fixed order, then v += dv. */

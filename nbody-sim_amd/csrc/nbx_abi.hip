@@ -676,40 +676,80 @@ int nbx_walk_counts(nbx_tree *t, float theta, int32_t *dev_n_acc, int32_t *dev_n
     return finish(t);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Counting walk with the current shape: per-body accepted / opened counts
+// (host, 2 n ints) and per-wave {records, batches, deepest stack} (host,
+// 3 ints per wave).
+int walk_profile(nbx_tree *t, float theta, std::vector<int> &counts, std::vector<int> &waves)
+{
+    const int n = t->T.n_total;
+    const int nw = walk_waves(t->T, t->walk_tg);
+    int *acc = nullptr, *ws = nullptr;
+    int rc = dmalloc(&acc, (size_t)2 * n);
+    if (rc) return rc;
+    if ((rc = dmalloc(&ws, (size_t)3 * nw)) != NBX_OK) {
+        (void)hipFree(acc);
+        return rc;
+    }
+    launch_walk_counts(t->T, theta * theta, t->walk_tg, acc, acc + n, ws, t->B, t->stream);
+    counts.resize(2 * (size_t)n);
+    waves.resize(3 * (size_t)nw);
+    hipError_t e = hipMemcpyAsync(counts.data(), acc, sizeof(int) * 2 * n,
+                                  hipMemcpyDeviceToHost, t->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(waves.data(), ws, sizeof(int) * 3 * nw, hipMemcpyDeviceToHost,
+                           t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+    (void)hipFree(acc);
+    (void)hipFree(ws);
+    if (e != hipSuccess) return fail(NBX_E_HIP, "walk profile: %s", hipGetErrorString(e));
+    return NBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int nbx_walk_stats(nbx_tree *t, float theta, double *out)
 {
     if (!t || !out) return fail(NBX_E_ARG, "null argument");
     if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk statistics before summarize");
-    const int n = t->T.n_total;
-    int *acc = nullptr;
-    unsigned long long *pop = nullptr;
-    int rc = dmalloc(&acc, (size_t)2 * n);
+    std::vector<int> h, w;
+    int rc = walk_profile(t, theta, h, w);
     if (rc) return rc;
-    if ((rc = dmalloc(&pop, 1)) != NBX_OK) {
-        (void)hipFree(acc);
-        return rc;
-    }
-    HIPCHK(hipMemsetAsync(pop, 0, sizeof(*pop), t->stream));
-    HIPCHK(hipMemsetAsync(t->B.err + 2, 0, sizeof(int), t->stream));
-    launch_walk_counts(t->T, theta * theta, t->walk_tg, acc, acc + n, pop, t->B, t->stream);
-    std::vector<int> h(2 * n);
-    unsigned long long hp = 0;
+    const int n = t->T.n_total, nw = (int)(w.size() / 3);
+    double sa = 0, so = 0, sp = 0;
     int depth = 0;
-    HIPCHK(hipMemcpyAsync(h.data(), acc, sizeof(int) * 2 * n, hipMemcpyDeviceToHost, t->stream));
-    HIPCHK(hipMemcpyAsync(&hp, pop, sizeof(hp), hipMemcpyDeviceToHost, t->stream));
-    HIPCHK(hipMemcpyAsync(&depth, t->B.err + 2, sizeof(int), hipMemcpyDeviceToHost, t->stream));
-    HIPCHK(hipStreamSynchronize(t->stream));
-    (void)hipFree(acc);
-    (void)hipFree(pop);
-    double sa = 0, so = 0;
     for (int i = 0; i < n; ++i) {
         sa += h[i];
         so += h[n + i];
     }
+    for (int k = 0; k < nw; ++k) {
+        sp += w[3 * k];
+        depth = depth > w[3 * k + 2] ? depth : w[3 * k + 2];
+    }
     out[0] = sa / n;
     out[1] = so / n;
-    out[2] = (double)hp / walk_waves(t->T, t->walk_tg);
+    out[2] = sp / nw;
     out[3] = depth;
+    return finish(t);
+}
+
+int nbx_walk_wave_stats(nbx_tree *t, float theta, int32_t *out, int32_t cap, int32_t *n_waves)
+{
+    if (!t || !n_waves) return fail(NBX_E_ARG, "null argument");
+    if (t->stage < kSummarized) return fail(NBX_E_STATE, "walk statistics before summarize");
+    const int nw = walk_waves(t->T, t->walk_tg);
+    *n_waves = nw;
+    if (!out) return NBX_OK;
+    if (cap < nw) return fail(NBX_E_ARG, "output holds %d waves, the walk has %d", cap, nw);
+    std::vector<int> h, w;
+    int rc = walk_profile(t, theta, h, w);
+    if (rc) return rc;
+    memcpy(out, w.data(), sizeof(int) * w.size());
     return finish(t);
 }
 

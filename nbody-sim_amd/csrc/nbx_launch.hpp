@@ -144,9 +144,10 @@ struct WalkArgs {
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
                  Bufs &B, hipStream_t st);
 // Per-body accepted / opened record counts of the gravity walk with the given
-// targets per wave; *popped (optional) accumulates the records the waves pop.
+// targets per wave; wstat (optional, 3 ints per wave): records popped,
+// batches, deepest stack of every wave.
 void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
-                        unsigned long long *popped, Bufs &B, hipStream_t st);
+                        int *wstat, Bufs &B, hipStream_t st);
 int walk_waves(const SegTable &T, int targets);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,

@@ -71,7 +71,7 @@ template <int TG>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     SegTable T, WaveMap M, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, int *n_acc,
-    int *n_open, int *err, unsigned long long *popped)
+    int *n_open, int *err, int *wstat)
 {
     __shared__ WaveLds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     const float4 b = bodies[valid ? pos : T.off[s]];
     CountVisitor v{b.x, b.y, b.z, theta2, pos, 0, 0};
     wave_walk<TG, false>(ra, rb, nullptr, T.noff[s], ballot64(valid && rr == 0), lds[wib], err, v,
-                         popped);
+                         wstat ? wstat + 3 * w : nullptr);
     const int na = slot_sum_i<TG>(v.n_acc), no = slot_sum_i<TG>(v.n_open);
     if (!valid || rr != 0) return;
     const int i = perm[pos];
@@ -345,7 +345,7 @@ void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, co
 
 template <int TG>
 static void launch_counts_tg(const SegTable &T, float theta2, int *n_acc, int *n_open,
-                             unsigned long long *popped, Bufs &B, hipStream_t st)
+                             int *wstat, Bufs &B, hipStream_t st)
 {
     WaveMap M;
     M.wave[0] = 0;
@@ -353,7 +353,7 @@ static void launch_counts_tg(const SegTable &T, float theta2, int *n_acc, int *n
         M.wave[s + 1] = M.wave[s] + (s < T.nseg ? cdiv(T.off[s + 1] - T.off[s], TG) : 0);
     hipLaunchKernelGGL(k_walk_counts<TG>, dim3(cdiv(M.wave[T.nseg], kWalkWaves)),
                        dim3(kWalkBlock), 0, st, T, M, B.rec_a, B.rec_b, B.bodies, B.vals[B.cur],
-                       theta2, n_acc, n_open, B.err, popped);
+                       theta2, n_acc, n_open, B.err, wstat);
 }
 
 int walk_waves(const SegTable &T, int targets)
@@ -364,13 +364,13 @@ int walk_waves(const SegTable &T, int targets)
 }
 
 void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
-                        unsigned long long *popped, Bufs &B, hipStream_t st)
+                        int *wstat, Bufs &B, hipStream_t st)
 {
     switch (targets) {
-    case 8: launch_counts_tg<8>(T, theta2, n_acc, n_open, popped, B, st); break;
-    case 16: launch_counts_tg<16>(T, theta2, n_acc, n_open, popped, B, st); break;
-    case 32: launch_counts_tg<32>(T, theta2, n_acc, n_open, popped, B, st); break;
-    default: launch_counts_tg<64>(T, theta2, n_acc, n_open, popped, B, st); break;
+    case 8: launch_counts_tg<8>(T, theta2, n_acc, n_open, wstat, B, st); break;
+    case 16: launch_counts_tg<16>(T, theta2, n_acc, n_open, wstat, B, st); break;
+    case 32: launch_counts_tg<32>(T, theta2, n_acc, n_open, wstat, B, st); break;
+    default: launch_counts_tg<64>(T, theta2, n_acc, n_open, wstat, B, st); break;
     }
 }
 

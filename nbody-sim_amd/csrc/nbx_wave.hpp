@@ -53,7 +53,8 @@ struct WaveLdsQ : WaveLds {
 // take identical decisions.
 __device__ inline float dist2(float dx, float dy, float dz)
 {
-    return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+#pragma clang fp contract(off)
+    return (dx * dx + dy * dy) + dz * dz;
 }
 
 // Staging with the record index of every slot (visitors that write per
@@ -87,7 +88,7 @@ struct Rec {
 template <int TG, bool QUAD, class Lds, class Visitor>
 __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
                                  const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
-                                 int *err, Visitor &vis, unsigned long long *popped = nullptr)
+                                 int *err, Visitor &vis, int *wstat = nullptr)
 {
     constexpr int LS = kWave / TG;
     constexpr uint64_t TMASK = TG == 64 ? ~0ull : ((1ull << TG) - 1ull);
@@ -97,28 +98,40 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
     // later reads, no cross-lane ordering is needed
     L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
     int sp = 1, sp_max = 1;
+    int n_pop = 0, n_batch = 0;  // records popped, batches (wave-uniform)
     while (sp > 0) {
         // take up to 64 records from the top blocks (from the end of a block)
         int nrec = 0, my_rec = -1;
         uint64_t my_mask = 0;
         const bool deep = sp > kBatchMaxSp;
-        while (sp > 0 && nrec < kWave) {
-            const StackEntry e = L.st[sp - 1];
-            const int count = rfl(e.count);
+        // the top entries are read in one parallel LDS access (lane j holds
+        // entry sp-1-j) and walked with lane reads, not one LDS round trip
+        // per entry
+        const int ne = min(sp, kWave);
+        StackEntry top{0, 0, 0u, 0u};
+        if (lane < ne) top = L.st[sp - 1 - lane];
+        int used = 0;
+        for (int j = 0; j < ne && nrec < kWave; ++j) {
+            const int count = __builtin_amdgcn_readlane(top.count, j);
+            const int first = __builtin_amdgcn_readlane(top.first, j);
+            const unsigned mlo = __builtin_amdgcn_readlane(top.mlo, j);
+            const unsigned mhi = __builtin_amdgcn_readlane(top.mhi, j);
             const int take = min(count, kWave - nrec);
             if (lane >= nrec && lane < nrec + take) {
-                my_rec = e.first + (count - take) + (lane - nrec);
-                my_mask = ((uint64_t)e.mhi << 32) | (uint64_t)e.mlo;
+                my_rec = first + (count - take) + (lane - nrec);
+                my_mask = ((uint64_t)mhi << 32) | (uint64_t)mlo;
             }
             if (take == count) {
-                --sp;
+                ++used;
             } else {
-                L.st[sp - 1].count = count - take;
+                L.st[sp - 1 - j].count = count - take;  // stays on top
             }
             nrec += take;
             if (deep) break;
         }
-        if (popped && lane == 0) atomicAdd(popped, (unsigned long long)nrec);
+        sp -= used;
+        n_pop += nrec;
+        ++n_batch;
         // one vector load per lane, staged in LDS
         if (my_rec >= 0) {
             L.a[lane] = ra[my_rec];
@@ -133,28 +146,40 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int k0 = 0; k0 < nrec; k0 += LS) {
-            // slots past the batch read a valid staging entry and are masked off
-            const int k = min(k0 + rr, nrec - 1);
-            const bool mine = (k0 + rr < nrec) && ((L.m[k] >> tt) & 1ull);
-            Rec r;
+        // Record steps, software-pipelined: the staging entries of the next
+        // step are read while this one is evaluated (the staging area does
+        // not change during the batch).  Slots past the batch read a valid
+        // entry and are masked off.
+        auto stage = [&](int k, Rec &r, uint64_t &m) {
             r.a = L.a[k];
             r.b = L.b[k];
+            m = L.m[k];
             if constexpr (wants_rec_id<Visitor>::value) r.id = L.id[k];
             if constexpr (QUAD) {
                 r.q0 = L.q[2 * k];
                 r.q1 = L.q[2 * k + 1];
             }
-            const bool open = vis.visit_masked(r, mine);
+        };
+        Rec cur;
+        uint64_t mcur;
+        stage(min(rr, nrec - 1), cur, mcur);
+        for (int k0 = 0; k0 < nrec; k0 += LS) {
+            Rec nxt;
+            uint64_t mnxt;
+            stage(min(k0 + LS + rr, nrec - 1), nxt, mnxt);
+            const bool mine = (k0 + rr < nrec) && ((mcur >> tt) & 1ull);
+            const bool open = vis.visit_masked(cur, mine);
             const uint64_t om = ballot64(open);
             if (om) {
 #pragma unroll
                 for (int q = 0; q < LS; ++q) {
                     const uint64_t mq = (om >> (q * TG)) & TMASK;
                     if (mq) {
-                        const int4 b = L.b[k0 + q];
+                        // every lane of slot q holds record k0 + q
+                        const int first = __builtin_amdgcn_readlane(cur.b.z, q * TG);
+                        const int count = __builtin_amdgcn_readlane(cur.b.w, q * TG);
                         if (sp < kGStack) {
-                            L.st[sp] = StackEntry{b.z, b.w, (unsigned)mq, (unsigned)(mq >> 32)};
+                            L.st[sp] = StackEntry{first, count, (unsigned)mq, (unsigned)(mq >> 32)};
                             ++sp;
                         } else if (lane == 0) {
                             atomicOr(err, kErrStack);
@@ -163,12 +188,19 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
                 }
                 sp_max = max(sp_max, sp);
             }
+            cur = nxt;
+            mcur = mnxt;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (lane == 0) atomicMax(err + 2, sp_max);  // deepest stack (diagnostic)
+    if (wstat && lane == 0) {  // per-wave work: records, batches, deepest stack
+        wstat[0] = n_pop;
+        wstat[1] = n_batch;
+        wstat[2] = sp_max;
+    }
 }
 
 // Sum of a per-lane value over the LS record slots of target tt.

@@ -81,7 +81,7 @@ EXPORTS = [
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
     "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
     "nbx_tree_set_radix_bits", "nbx_tree_set_adjoint", "nbx_adj_force", "nbx_adj_step",
-    "nbx_adj_init_forces",
+    "nbx_adj_init_forces", "nbx_walk_wave_stats",
 ]
 
 _lib = None
@@ -125,6 +125,7 @@ def lib():
             "nbx_walk_stats": [vp, f32, C.POINTER(C.c_double)],
             "nbx_tree_set_radix_bits": [vp, i32],
             "nbx_tree_set_adjoint": [vp, i32],
+            "nbx_walk_wave_stats": [vp, f32, vp, i32, C.POINTER(i32)],
             "nbx_adj_force": [vp, P, f32, f32, f32, C.POINTER(nbx_disk_params), f32, vp, vp,
                               C.POINTER(C.c_double)],
             "nbx_adj_step": [vp, P, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params),
@@ -427,6 +428,17 @@ def walk_stats(t, theta):
     out = (C.c_double * 4)()
     _check(lib().nbx_walk_stats(t.h, theta, out), "walk_stats")
     return dict(acc=out[0], open=out[1], records_per_wave=out[2], max_stack=out[3])
+
+
+def walk_wave_stats(t, theta):
+    """Per-wavefront work of the theta walk on the last summarized tree:
+    int32 array (waves, 3) of {records evaluated, batches, deepest stack}."""
+    nw = C.c_int32()
+    _check(lib().nbx_walk_wave_stats(t.h, theta, None, 0, C.byref(nw)), "walk_wave_stats")
+    out = np.zeros((nw.value, 3), np.int32)
+    _check(lib().nbx_walk_wave_stats(t.h, theta, out.ctypes.data_as(C.c_void_p), nw.value,
+                                     C.byref(nw)), "walk_wave_stats")
+    return out
 
 
 def fof_into(t, p, link_length, labels):
