@@ -226,6 +226,66 @@ int nbx_adj_init_forces(nbx_tree *t, const nbx_particles *s0, const nbx_step_par
                         const nbx_disk_params *dp, const double *abar, double *xbar,
                         double *pgrad);
 
+/* ---- run support on the host: config, disk, outputs ---------------------- */
+/* Config (Config.h:5-35): the reference's 19 fields with its defaults, plus
+ * generator fields with no reference counterpart: disk_seed (initDisk's
+ * seed argument, default 42), and e_sigma / i_sigma (< 0, the default: the
+ * reference's 1 % Gaussian velocity noise, DiskInit.cpp:55-60; >= 0:
+ * Rayleigh eccentricities / inclinations of that width instead). */
+typedef struct nbx_config {
+    int32_t n_particles;
+    float disk_r_min, disk_r_max, disk_h_factor;
+    float particle_mass_msun, star_mass_msun, softening_AU, theta;
+    float collision_radius_factor, restitution, gas_drag_rate;
+    float dt, total_time;
+    int32_t output_every_n_steps;
+    char output_dir[512];
+    int32_t perturber_enabled;
+    float perturber_mass_mjup, perturber_radius_AU, perturber_phase0;
+    uint32_t disk_seed;
+    float e_sigma, i_sigma;
+} nbx_config;
+/* Defaults of Config.h:5-35. */
+int nbx_config_default(nbx_config *cfg);
+/* loadConfig (Config.cpp:5-55): JSON object, missing keys keep their
+ * defaults, numbers parsed as double then narrowed, the checks of
+ * Config.cpp:40-52 (NBX_E_ARG with the reference's message). */
+int nbx_config_load(const char *path, nbx_config *cfg);
+/* particlePhysicalRadius (DiskInit.cpp:10-16) of cfg->particle_mass_msun. */
+float nbx_particle_radius(const nbx_config *cfg);
+/* initDisk (DiskInit.cpp:18-67) into HOST arrays of cfg->n_particles bodies
+ * (the same distribution; the random stream is this library's counter-based
+ * generator keyed by seed, not std::mt19937). */
+int nbx_init_disk(const nbx_config *cfg, uint32_t seed, nbx_particles *host);
+
+/* Aggregate row (Aggregate.h:4-15 without the member list). */
+typedef struct nbx_aggregate {
+    int32_t id, n_particles;
+    float mass_msun, cx, cy, cz, vx, vy, vz, dist_center;
+} nbx_aggregate;
+/* OutputWriter (OutputWriter.h / OutputWriter.cpp): frames.bin version 2
+ * and stats.csv in `dir`.  Host arrays throughout. */
+typedef struct nbx_writer nbx_writer;
+/* OutputWriter::OutputWriter: create both files, write both headers. */
+int nbx_writer_open(nbx_writer **w, const char *dir, int32_t n_particles, float dt,
+                    int32_t output_every_n_steps);
+/* writeFrame: time, x[n], y[n], z[n], agg_size[n] (uint16). */
+int nbx_writer_frame(nbx_writer *w, const nbx_particles *host, float sim_time_T0,
+                     const uint16_t *agg_sizes);
+/* writeStats: one row per aggregate. */
+int nbx_writer_stats(nbx_writer *w, int32_t frame, float sim_time_T0, int32_t n_aggregates,
+                     const nbx_aggregate *aggs);
+/* writePerturberStats: the agg_id = -1 row. */
+int nbx_writer_perturber(nbx_writer *w, int32_t frame, float sim_time_T0, int32_t n_aggregates,
+                         float px, float py, float pz, float mass_msun, float omega);
+/* finalize: patch n_frames in the header and flush. */
+int nbx_writer_finalize(nbx_writer *w, int32_t frames_written);
+int nbx_writer_close(nbx_writer *w);
+/* No reference counterpart in the engine: the visualizer's perturber.json
+ * sidecar (PerturberRenderer.cs:67-73 field names). */
+int nbx_write_perturber_sidecar(const char *dir, int32_t enabled, float mass_msun, float r_AU,
+                                float phase0_rad, float omega);
+
 /* ---- diagnostics --------------------------------------------------------- */
 /* Tree statistics of the last build: number of octree nodes and maximum
  * depth per disk (host arrays of length n_seg, may be NULL). */

@@ -203,4 +203,82 @@ inline std::vector<Aggregate> detectAggregates(OctreeData &t, const ParticleData
     return out;
 }
 
+// ---- run support (Config.h, DiskInit.h, OutputWriter.h) -------------------
+using Config = nbx_config;
+
+inline Config loadConfig(const std::string &path)  // Config.cpp:5-55
+{
+    Config c;
+    check(nbx_config_load(path.c_str(), &c), "loadConfig");
+    return c;
+}
+
+inline float particlePhysicalRadius(const Config &cfg) { return nbx_particle_radius(&cfg); }
+
+inline ParticleData initDisk(const Config &cfg, unsigned int seed = 42)  // DiskInit.h:9
+{
+    ParticleData p = allocateParticlesCPU(cfg.n_particles);
+    try {
+        check(nbx_init_disk(&cfg, seed, &p), "initDisk");
+    } catch (...) {
+        freeParticlesCPU(p);
+        throw;
+    }
+    return p;
+}
+
+inline Perturber makePerturber(const Config &cfg)  // Perturber.h:39-57
+{
+    return makePerturber(cfg.perturber_enabled != 0, cfg.perturber_mass_mjup,
+                         cfg.perturber_radius_AU, cfg.perturber_phase0, cfg.star_mass_msun);
+}
+
+constexpr float T0_TO_YEARS = 1.0f / (2.0f * 3.14159265f);  // OutputWriter.h:10-12
+constexpr float MSUN_TO_MEARTH = 332946.0f;
+
+class OutputWriter {  // OutputWriter.h:14-49
+public:
+    OutputWriter(const std::string &output_dir, const Config &cfg) : n_(cfg.n_particles)
+    {
+        check(nbx_writer_open(&w_, output_dir.c_str(), cfg.n_particles, cfg.dt,
+                              cfg.output_every_n_steps),
+              "OutputWriter");
+    }
+    ~OutputWriter() { nbx_writer_close(w_); }
+    OutputWriter(const OutputWriter &) = delete;
+    OutputWriter &operator=(const OutputWriter &) = delete;
+    void writeFrame(const ParticleData &cpu, float sim_time_T0, const std::vector<uint16_t> &agg)
+    {
+        if ((int)agg.size() != n_)
+            throw std::runtime_error(
+                "OutputWriter::writeFrame: agg_sizes_per_particle size mismatch");
+        check(nbx_writer_frame(w_, &cpu, sim_time_T0, agg.data()), "writeFrame");
+    }
+    void writeStats(const std::vector<Aggregate> &aggs, int frame_idx, float sim_time_T0)
+    {
+        std::vector<nbx_aggregate> rows;
+        rows.reserve(aggs.size());
+        for (const Aggregate &a : aggs)
+            rows.push_back(nbx_aggregate{a.id, a.n_particles, a.mass_msun, a.cx, a.cy, a.cz,
+                                         a.vx, a.vy, a.vz, a.dist_center});
+        check(nbx_writer_stats(w_, frame_idx, sim_time_T0, (int)rows.size(), rows.data()),
+              "writeStats");
+    }
+    void writePerturberStats(int frame_idx, float sim_time_T0, int n_aggregates, float px,
+                             float py, float pz, float mass_msun, float omega)
+    {
+        check(nbx_writer_perturber(w_, frame_idx, sim_time_T0, n_aggregates, px, py, pz,
+                                   mass_msun, omega),
+              "writePerturberStats");
+    }
+    void finalize(int total_frames_written)
+    {
+        check(nbx_writer_finalize(w_, total_frames_written), "finalize");
+    }
+
+private:
+    nbx_writer *w_ = nullptr;
+    int n_ = 0;
+};
+
 }  // namespace nbx_ref

@@ -46,6 +46,11 @@ const int kShapes[4] = {64, 32, 16, 8};
 
 }  // namespace
 
+namespace nbx {
+// Shared with the host-side run support (nbx_io.cpp).
+void set_last_error(const char *msg) { g_err = msg; }
+}  // namespace nbx
+
 struct nbx_tree {
     SegTable T{};
     Bufs B{};

@@ -25,7 +25,8 @@ _PKG = os.path.dirname(_HERE)
 _REPO = os.path.dirname(_PKG)
 CSRC = os.path.join(_PKG, "csrc")
 LIB_PATH = os.path.join(_HERE, "libnbx.so")
-HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_adjoint.hip", "nbx_abi.hip"]
+RUN_PATH = os.path.join(_HERE, "nbx_run")
+HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_adjoint.hip", "nbx_abi.hip", "nbx_io.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep hand-written float code on single-lane VALU ops (no v_pk_*_f32)
@@ -44,8 +45,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB_PATH)
+    # the C++ driver of src/main.cu, linked against the library beside it
+    run = ["g++", "-O2", "-std=c++17", "-Wall", "-o", RUN_PATH + ".tmp",
+           os.path.join(CSRC, "nbx_run.cpp"), "-L", _HERE, "-lnbx", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(run, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(RUN_PATH + ".tmp", RUN_PATH)
     if verbose:
-        print("built", LIB_PATH)
+        print("built", LIB_PATH, "and", RUN_PATH)
     return LIB_PATH
 
 
@@ -54,6 +62,26 @@ class nbx_particles(C.Structure):
                 ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
                 ("ax", C.c_void_p), ("ay", C.c_void_p), ("az", C.c_void_p),
                 ("mass", C.c_void_p), ("id", C.c_void_p), ("n", C.c_int32)]
+
+
+class nbx_config(C.Structure):
+    """Config (Config.h:5-35) plus the generator fields disk_seed, e_sigma, i_sigma."""
+    _fields_ = [("n_particles", C.c_int32), ("disk_r_min", C.c_float),
+                ("disk_r_max", C.c_float), ("disk_h_factor", C.c_float),
+                ("particle_mass_msun", C.c_float), ("star_mass_msun", C.c_float),
+                ("softening_AU", C.c_float), ("theta", C.c_float),
+                ("collision_radius_factor", C.c_float), ("restitution", C.c_float),
+                ("gas_drag_rate", C.c_float), ("dt", C.c_float), ("total_time", C.c_float),
+                ("output_every_n_steps", C.c_int32), ("output_dir", C.c_char * 512),
+                ("perturber_enabled", C.c_int32), ("perturber_mass_mjup", C.c_float),
+                ("perturber_radius_AU", C.c_float), ("perturber_phase0", C.c_float),
+                ("disk_seed", C.c_uint32), ("e_sigma", C.c_float), ("i_sigma", C.c_float)]
+
+
+class nbx_aggregate(C.Structure):
+    _fields_ = [("id", C.c_int32), ("n_particles", C.c_int32), ("mass_msun", C.c_float),
+                ("cx", C.c_float), ("cy", C.c_float), ("cz", C.c_float), ("vx", C.c_float),
+                ("vy", C.c_float), ("vz", C.c_float), ("dist_center", C.c_float)]
 
 
 class nbx_step_params(C.Structure):
@@ -81,7 +109,10 @@ EXPORTS = [
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
     "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
     "nbx_tree_set_radix_bits", "nbx_tree_set_adjoint", "nbx_adj_force", "nbx_adj_step",
-    "nbx_adj_init_forces", "nbx_walk_wave_stats",
+    "nbx_adj_init_forces", "nbx_walk_wave_stats", "nbx_config_default", "nbx_config_load",
+    "nbx_particle_radius", "nbx_init_disk", "nbx_writer_open", "nbx_writer_frame",
+    "nbx_writer_stats", "nbx_writer_perturber", "nbx_writer_finalize", "nbx_writer_close",
+    "nbx_write_perturber_sidecar",
 ]
 
 _lib = None
@@ -126,6 +157,16 @@ def lib():
             "nbx_tree_set_radix_bits": [vp, i32],
             "nbx_tree_set_adjoint": [vp, i32],
             "nbx_walk_wave_stats": [vp, f32, vp, i32, C.POINTER(i32)],
+            "nbx_config_default": [C.POINTER(nbx_config)],
+            "nbx_config_load": [C.c_char_p, C.POINTER(nbx_config)],
+            "nbx_init_disk": [C.POINTER(nbx_config), C.c_uint32, P],
+            "nbx_writer_open": [C.POINTER(vp), C.c_char_p, i32, f32, i32],
+            "nbx_writer_frame": [vp, P, f32, vp],
+            "nbx_writer_stats": [vp, i32, f32, i32, C.POINTER(nbx_aggregate)],
+            "nbx_writer_perturber": [vp, i32, f32, i32, f32, f32, f32, f32, f32],
+            "nbx_writer_finalize": [vp, i32],
+            "nbx_writer_close": [vp],
+            "nbx_write_perturber_sidecar": [C.c_char_p, i32, f32, f32, f32, f32],
             "nbx_adj_force": [vp, P, f32, f32, f32, C.POINTER(nbx_disk_params), f32, vp, vp,
                               C.POINTER(C.c_double)],
             "nbx_adj_step": [vp, P, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params),
@@ -137,6 +178,8 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        L.nbx_particle_radius.argtypes = [C.POINTER(nbx_config)]
+        L.nbx_particle_radius.restype = C.c_float
         _lib = L
     return _lib
 
@@ -581,3 +624,92 @@ def phase_ms(t, phase):
     v = C.c_float()
     _check(lib().nbx_phase_ms(t.h, phase.encode(), C.byref(v)), "phase_ms")
     return v.value
+
+
+# ---------------------------------------------------------------------------
+# Run support on the host (Config.h, DiskInit.h, OutputWriter.h)
+# ---------------------------------------------------------------------------
+def config_default():
+    c = nbx_config()
+    _check(lib().nbx_config_default(C.byref(c)), "config_default")
+    return c
+
+
+def loadConfig(path):
+    """loadConfig (Config.cpp:5-55); raises RuntimeError with its messages."""
+    c = nbx_config()
+    _check(lib().nbx_config_load(os.fsencode(path), C.byref(c)), "loadConfig")
+    return c
+
+
+def config_particle_radius(cfg):
+    return lib().nbx_particle_radius(C.byref(cfg))
+
+
+def _host_particles(n):
+    arrays = {k: np.zeros(n, np.float32) for k in FIELDS}
+    arrays["id"] = np.zeros(n, np.int32)
+    s = nbx_particles()
+    for k in FIELDS:
+        setattr(s, k, arrays[k].ctypes.data)
+    s.id = arrays["id"].ctypes.data
+    s.n = n
+    return s, arrays
+
+
+def initDisk(cfg, seed=None):
+    """initDisk (DiskInit.cpp:18-67) on the host: dict of float32 arrays."""
+    s, arrays = _host_particles(cfg.n_particles)
+    _check(lib().nbx_init_disk(C.byref(cfg), cfg.disk_seed if seed is None else seed,
+                               C.byref(s)), "initDisk")
+    return arrays
+
+
+class OutputWriter:
+    """OutputWriter (OutputWriter.h:14-49) over nbx_writer_*."""
+
+    def __init__(self, output_dir, n_particles, dt, output_every_n_steps):
+        self.h = C.c_void_p()
+        self.n = int(n_particles)
+        _check(lib().nbx_writer_open(C.byref(self.h), os.fsencode(output_dir), self.n, dt,
+                                     output_every_n_steps), "OutputWriter")
+
+    def writeFrame(self, host, sim_time_T0, agg_sizes):
+        s = nbx_particles()
+        keep = {}
+        for k in ("x", "y", "z"):
+            keep[k] = np.ascontiguousarray(host[k], np.float32)
+            setattr(s, k, keep[k].ctypes.data)
+        s.n = len(keep["x"])
+        a = np.ascontiguousarray(agg_sizes, np.uint16)
+        if len(a) != self.n:
+            raise RuntimeError("OutputWriter::writeFrame: agg_sizes_per_particle size mismatch")
+        _check(lib().nbx_writer_frame(self.h, C.byref(s), sim_time_T0,
+                                      a.ctypes.data_as(C.c_void_p)), "writeFrame")
+
+    def writeStats(self, aggregates, frame_idx, sim_time_T0):
+        rows = (nbx_aggregate * max(1, len(aggregates)))()
+        for k, a in enumerate(aggregates):
+            rows[k] = nbx_aggregate(a.id, a.n_particles, a.mass_msun, a.cx, a.cy, a.cz,
+                                    a.vx, a.vy, a.vz, a.dist_center)
+        _check(lib().nbx_writer_stats(self.h, frame_idx, sim_time_T0, len(aggregates), rows),
+               "writeStats")
+
+    def writePerturberStats(self, frame_idx, sim_time_T0, n_aggregates, px, py, pz, mass_msun,
+                            omega):
+        _check(lib().nbx_writer_perturber(self.h, frame_idx, sim_time_T0, n_aggregates, px, py,
+                                          pz, mass_msun, omega), "writePerturberStats")
+
+    def finalize(self, total_frames_written):
+        _check(lib().nbx_writer_finalize(self.h, total_frames_written), "finalize")
+
+    def close(self):
+        if self.h.value:
+            lib().nbx_writer_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -111,7 +111,7 @@ def lib():
 
 
 def ref_lib():
-    """The reference's own FoF / disk-init code, or None when not built."""
+    """The reference's own FoF / disk-init / output-writer code, or None when not built."""
     global _ref
     if _ref is None and os.path.exists(_REF_PATH):
         R = C.CDLL(_REF_PATH)
@@ -120,6 +120,11 @@ def ref_lib():
             C.c_float, C.c_float, _i32p, C.c_int, _i32p, _f32p]
         R.ref_particle_physical_radius.restype = C.c_float
         R.ref_particle_physical_radius.argtypes = [C.c_float]
+        if hasattr(R, "ref_write_run"):
+            R.ref_write_run.restype = C.c_int
+            R.ref_write_run.argtypes = [C.c_char_p, C.c_int, C.c_float, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_int] + [C.c_float] * 5
         _ref = R
     return _ref
 

@@ -1,11 +1,12 @@
 // ref_shim.cpp -- C entry points over the reference's own host-side C++
-// (Aggregate.cpp, DiskInit.cpp), compiled from /root/reference by
+// (Aggregate.cpp, DiskInit.cpp, OutputWriter.cpp), compiled from /root/reference by
 // oracle/Makefile into oracle/_ref/libref.so.  TEST INFRASTRUCTURE ONLY:
 // used to check the oracle's FoF / disk-init restatements, never by the
 // product path.
 #include "Aggregate.h"
 #include "Config.h"
 #include "DiskInit.h"
+#include "OutputWriter.h"
 #include "ParticleData.h"
 
 #include <algorithm>
@@ -61,6 +62,48 @@ float ref_particle_physical_radius(float particle_mass) {
     Config cfg;
     cfg.particle_mass_msun = particle_mass;
     return particlePhysicalRadius(cfg);
+}
+
+// OutputWriter (OutputWriter.h:14-49) over a scripted run: n_frames frames
+// of positions xyz[f][3][n] at times[f] with agg_sizes[f][n], after each
+// frame the same n_agg aggregate rows (ai[k] = {id, n_particles}, af[k] =
+// {mass, cx, cy, cz, vx, vy, vz, dist}) and, if pert, the perturber row.
+int ref_write_run(const char* dir, int n, float dt, int every, int n_frames,
+                  const float* xyz, const float* times, const uint16_t* agg_sizes,
+                  int n_agg, const int32_t* ai, const float* af,
+                  int pert, float px, float py, float pz, float pmass, float omega) {
+    Config cfg;
+    cfg.n_particles = n;
+    cfg.dt = dt;
+    cfg.output_every_n_steps = every;
+    std::vector<Aggregate> aggs(n_agg);
+    for (int k = 0; k < n_agg; ++k) {
+        Aggregate& a = aggs[k];
+        a.id = ai[2 * k];
+        a.n_particles = ai[2 * k + 1];
+        const float* f = af + 8 * k;
+        a.mass_msun = f[0];
+        a.cx = f[1]; a.cy = f[2]; a.cz = f[3];
+        a.vx = f[4]; a.vy = f[5]; a.vz = f[6];
+        a.dist_center = f[7];
+    }
+    ParticleData p = allocateParticlesCPU(n);
+    {
+        OutputWriter w(dir, cfg);
+        for (int fr = 0; fr < n_frames; ++fr) {
+            const float* b = xyz + (size_t)fr * 3 * n;
+            std::memcpy(p.x, b, n * sizeof(float));
+            std::memcpy(p.y, b + n, n * sizeof(float));
+            std::memcpy(p.z, b + 2 * n, n * sizeof(float));
+            std::vector<uint16_t> sizes(agg_sizes + (size_t)fr * n, agg_sizes + (size_t)(fr + 1) * n);
+            w.writeFrame(p, times[fr], sizes);
+            w.writeStats(aggs, fr, times[fr]);
+            if (pert) w.writePerturberStats(fr, times[fr], n_agg, px, py, pz, pmass, omega);
+        }
+        w.finalize(n_frames);
+    }
+    freeParticlesCPU(p);
+    return 0;
 }
 
 }  // extern "C"
