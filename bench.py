@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--no-window", action="store_true",
                     help="skip the differentiable-window leg")
     ap.add_argument("--window-steps", type=int, default=200)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 1M-body precision study")
+    ap.add_argument("--precision-steps", type=int, default=500)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run)")
@@ -198,6 +201,8 @@ def main():
         out["batched_sweep"] = batched_sweep(nbx, np, torch, make_disk, args)
     if not args.no_window:
         out["diff_window"] = diff_window(nbx, np, torch, make_disk, args)
+    if not args.no_precision:
+        out["precision_study"] = precision_study(nbx, np, torch, make_disk, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024), r_coll,
                                            10.0, args.cpu_steps)
@@ -312,36 +317,114 @@ def diff_window(nbx, np, torch, make_disk, args):
             "dL_dv0_norm": torch.linalg.norm(v0g.grad.double()).item()}
 
 
-def force_accuracy(nbx, t, p, n, n_targets=1024):
-    """Self-gravity error of the GPU walk against an FP64 direct sum on the
-    GPU (torch float64) over seeded targets, and the mean interaction
-    counts of the walk estimated on those targets."""
+def fp64_direct(pos, m, tg, eps):
+    """FP64 softened direct-sum self-gravity at the targets tg (torch, GPU)."""
+    import torch
+    chunk = max(1024, (1 << 26) // len(tg))
+    acc = torch.zeros(len(tg), 3, dtype=torch.float64, device="cuda")
+    pt = pos[tg]
+    for c0 in range(0, pos.shape[0], chunk):
+        src = pos[c0:c0 + chunk]
+        dd = src[None, :, :] - pt[:, None, :]
+        r2 = (dd * dd).sum(-1) + eps * eps
+        inv = m[c0:c0 + chunk][None, :] / (r2 * r2.sqrt())
+        same = (torch.arange(c0, c0 + src.shape[0], device="cuda")[None, :] == tg[:, None])
+        acc += (dd * inv.masked_fill(same, 0.0)[..., None]).sum(1)
+    return acc
+
+
+def fp64_reference(p, targets, eps=1e-3):
+    import torch
+    pos = torch.stack([p.x, p.y, p.z], 1).double()
+    return fp64_direct(pos, p.mass.double(), targets, eps)
+
+
+def walk_errors(nbx, t, p, n, theta, targets, ref, quad=False, eps=1e-3):
+    """Relative error of the GPU self-gravity and of the total acceleration
+    (star + self-gravity) against FP64 (ref: fp64_reference) at the targets."""
     import numpy as np
     import torch
     sa = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    if quad:
+        nbx.set_multipole(t, 2)
     nbx.launchBBoxKernel(t, p)
     nbx.launchBuildKernel(t, p)
     nbx.launchSummarizeKernel(t)
-    nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0, self_acc=sa)
-    g = torch.stack(sa, 1).double()
-    pos = torch.stack([p.x, p.y, p.z], 1).double()
-    m = p.mass.double()
+    nbx.launchForceKernel(t, p, theta, eps, 1.0, self_acc=sa)
+    if quad:
+        nbx.set_multipole(t, 1)
+    g = torch.stack(sa, 1).double()[targets]
+    tot = torch.stack([p.ax, p.ay, p.az], 1).double()[targets]
+    pt = torch.stack([p.x, p.y, p.z], 1).double()[targets]
+    r2 = (pt * pt).sum(1, keepdim=True) + eps * eps
+    ref_tot = ref - pt / (r2 * r2.sqrt())
+    es = ((g - ref).norm(dim=1) / ref.norm(dim=1)).cpu().numpy()
+    et = ((tot - ref_tot).norm(dim=1) / ref_tot.norm(dim=1)).cpu().numpy()
+    q = lambda e: {"median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+    return {"self_gravity": q(es), "total_acceleration": q(et)}
+
+
+def force_accuracy(nbx, t, p, n, n_targets=1024):
+    """Error of the GPU walk against an FP64 direct sum (torch float64 on the
+    GPU) over seeded targets: the reference rule (theta 0.5, monopoles) and
+    the accuracy option (theta 0.25 with quadrupoles); and the mean
+    interaction counts of the theta 0.5 walk (exact per-body counts)."""
+    import numpy as np
+    import torch
     tg = torch.from_numpy(np.sort(np.random.default_rng(0).choice(n, n_targets, replace=False)))
     tg = tg.cuda()
-    acc = torch.zeros(n_targets, 3, dtype=torch.float64, device="cuda")
-    for c0 in range(0, n, 16384):
-        src = pos[c0:c0 + 16384]
-        dd = src[None, :, :] - pos[tg][:, None, :]
-        r2 = (dd * dd).sum(-1) + 1e-6
-        inv = m[c0:c0 + 16384][None, :] / (r2 * r2.sqrt())
-        same = (torch.arange(c0, c0 + src.shape[0], device="cuda")[None, :] == tg[:, None])
-        inv = inv.masked_fill(same, 0.0)
-        acc += (dd * inv[..., None]).sum(1)
-    e = ((g[tg] - acc).norm(dim=1) / acc.norm(dim=1)).cpu().numpy()
-    counts = nbx_counts(nbx, t, p, n)
-    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)),
-            "targets": n_targets, "reference_rule": "h/r_com < theta (BHTree.cu:577)",
+    ref = fp64_reference(p, tg)
+    ref_rule = walk_errors(nbx, t, p, n, 0.5, tg, ref)
+    counts = nbx_counts(nbx, t, p, n)  # on the tree of the theta 0.5 walk
+    accurate = walk_errors(nbx, t, p, n, 0.25, tg, ref, quad=True)
+    return {"targets": n_targets, "reference_rule": "h/r_com < theta (BHTree.cu:577)",
+            "theta_0.5_monopole": ref_rule, "theta_0.25_quadrupole": accurate,
             "I_acc": counts[0], "I_open": counts[1]}
+
+
+def precision_study(nbx, np, torch, make_disk, args):
+    """Force-only precision study on a 1,048,576-body disk (r 0.3-10 AU):
+    walk error against FP64 for theta 0.3 / 0.5 / 0.7 on 16,384 seeded
+    targets, and the drift of the FP64 energy (kinetic + star potential) and
+    angular momentum over 500 gravity-only steps with the float KDK and with
+    the compensated FP64 KDK (forces FP32 in both)."""
+    n = 1 << 20
+    d = make_disk(n, DISK_MASS, 0.3, 10.0, seed=99)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(n)
+    tg = torch.from_numpy(np.sort(np.random.default_rng(1).choice(n, 16384, replace=False)))
+    tg = tg.cuda()
+    out = {"n": n, "targets": 16384, "theta": {}}
+    ref = fp64_reference(p, tg)
+    for th in (0.3, 0.5, 0.7):
+        e = walk_errors(nbx, t, p, n, th, tg, ref)
+        nbx.launchBBoxKernel(t, p)
+        nbx.launchBuildKernel(t, p)
+        nbx.launchSummarizeKernel(t)
+        e["walk_ms"] = nbx.tune_walk(t, th, 1e-3, 1.0)[nbx.walk_shape(t)]
+        out["theta"][str(th)] = e
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, gas_drag_rate=0.0,
+                         collisions=False, use_graph=True)
+    dp = nbx.disk_params(0.0, None)
+    steps = args.precision_steps
+    for mode in ("float", "compensated"):
+        q = nbx.to_gpu(d)
+        t2 = nbx.allocateOctree(n)
+        nbx.set_precision(t2, mode == "compensated")
+        nbx.init_forces(t2, q, sp, dp)
+        d0 = nbx.diagnostics(t2, q)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nbx.step(t2, q, sp, dp, 0, steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        d1 = nbx.diagnostics(t2, q)
+        e0, e1 = d0["kinetic"] + d0["star_potential"], d1["kinetic"] + d1["star_potential"]
+        out[mode] = {"steps": steps, "ms_per_step": ms,
+                     "energy_rel_drift": (e1 - e0) / abs(e0),
+                     "Ly_rel_drift": (d1["L"][1] - d0["L"][1]) / abs(d0["L"][1])}
+        t2.free()
+    return out
 
 
 def nbx_counts(nbx, t, p, n):
