@@ -22,14 +22,16 @@ constexpr int kWalkWaves = 1;   // one wavefront per block: walks never synchron
 constexpr int kWalkBlock = kWave * kWalkWaves;
 constexpr int kGStack = kStackCap;  // gravity stack entries per wave
 // Records of a popped batch are evaluated before their children are pushed,
-// so one batch of 64 records can add up to 64 entries.  Batches are taken
-// only while that cannot pass the capacity; deeper stacks pop one entry at a
-// time (depth-first), which adds at most 7 entries per octree level, and a
+// and the next batch is popped while the current one is evaluated, so up to
+// two batches of 64 records (128 entries) can be pushed after a batched pop.
+// Batches are taken only while that cannot pass the capacity; deeper stacks
+// pop one entry at a time (depth-first), which adds at most 7 entries per
+// octree level (plus the 8 children of the entry in flight), and a
 // root-to-leaf path has at most 22 levels (21 digits and a bucket of
 // identical keys, whose children are leaves).
-constexpr int kDeepMargin = 7 * 22;
-constexpr int kBatchMaxSp = kGStack - kDeepMargin - kWave;
-static_assert(kBatchMaxSp > 100, "stack too small for batched pops at theta = 0.5");
+constexpr int kDeepMargin = 7 * 22 + 8;
+constexpr int kBatchMaxSp = kGStack - kDeepMargin - 2 * kWave;
+static_assert(kBatchMaxSp >= 90, "stack too small for batched pops at theta = 0.5");
 constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
 
 struct StackEntry {
@@ -99,14 +101,14 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
     L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
     int sp = 1, sp_max = 1;
     int n_pop = 0, n_batch = 0;  // records popped, batches (wave-uniform)
-    while (sp > 0) {
-        // take up to 64 records from the top blocks (from the end of a block)
-        int nrec = 0, my_rec = -1;
-        uint64_t my_mask = 0;
+    // Take up to 64 records from the top blocks (from the end of a block).
+    // The top entries are read in one parallel LDS access (lane j holds
+    // entry sp-1-j) and walked with lane reads.
+    auto pop = [&](int &nrec, int &my_rec, uint64_t &my_mask) {
+        nrec = 0;
+        my_rec = -1;
+        my_mask = 0;
         const bool deep = sp > kBatchMaxSp;
-        // the top entries are read in one parallel LDS access (lane j holds
-        // entry sp-1-j) and walked with lane reads, not one LDS round trip
-        // per entry
         const int ne = min(sp, kWave);
         StackEntry top{0, 0, 0u, 0u};
         if (lane < ne) top = L.st[sp - 1 - lane];
@@ -131,21 +133,43 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
         }
         sp -= used;
         n_pop += nrec;
-        ++n_batch;
-        // one vector load per lane, staged in LDS
-        if (my_rec >= 0) {
-            L.a[lane] = ra[my_rec];
-            L.b[lane] = rb[my_rec];
-            L.m[lane] = my_mask;
-            if constexpr (wants_rec_id<Visitor>::value) L.id[lane] = my_rec;
+        n_batch += nrec > 0;
+    };
+    // The record loads of the next batch are in flight (in registers) while
+    // the current batch is evaluated: the next batch is popped from what the
+    // stack holds before the current batch pushes its children.
+    int nrec_n, rec_n;
+    uint64_t mask_n;
+    float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gq0 = ga, gq1 = ga;
+    int4 gb = make_int4(0, 0, 0, 0);
+    auto fetch = [&]() {
+        pop(nrec_n, rec_n, mask_n);
+        if (rec_n >= 0) {
+            ga = ra[rec_n];
+            gb = rb[rec_n];
             if constexpr (QUAD) {
-                L.q[2 * lane] = rq[2 * my_rec];
-                L.q[2 * lane + 1] = rq[2 * my_rec + 1];
+                gq0 = rq[2 * rec_n];
+                gq1 = rq[2 * rec_n + 1];
+            }
+        }
+    };
+    fetch();
+    while (nrec_n > 0) {
+        const int nrec = nrec_n;
+        if (rec_n >= 0) {  // stage the batch that arrived
+            L.a[lane] = ga;
+            L.b[lane] = gb;
+            L.m[lane] = mask_n;
+            if constexpr (wants_rec_id<Visitor>::value) L.id[lane] = rec_n;
+            if constexpr (QUAD) {
+                L.q[2 * lane] = gq0;
+                L.q[2 * lane + 1] = gq1;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        fetch();
         // Record steps, software-pipelined: the staging entries of the next
         // step are read while this one is evaluated (the staging area does
         // not change during the batch).  Slots past the batch read a valid
@@ -194,6 +218,7 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (nrec_n == 0 && sp > 0) fetch();  // the stack was empty when prefetching
     }
     if (lane == 0) atomicMax(err + 2, sp_max);  // deepest stack (diagnostic)
     if (wstat && lane == 0) {  // per-wave work: records, batches, deepest stack
