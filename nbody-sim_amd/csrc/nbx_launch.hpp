@@ -72,7 +72,7 @@ constexpr int kScanWords = 4 + 4 + 6 + 6;
 // bit pattern of a look-back word not yet written (a NaN no sum produces)
 constexpr unsigned long long kScanEmpty = 0x7ff8dead0badf00dull;
 constexpr int kHistInts = 6 * 2048;  // histograms of the widest setting (6 x 11-bit)
-constexpr int kSweepItems = 8;
+constexpr int kSweepItems = 4;
 constexpr int kSweepTile = kBlock * kSweepItems;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
