@@ -28,7 +28,7 @@ struct Bufs {
     // pass): tile aggregates and inclusive tile prefixes of both scans
     double *scanw = nullptr;     // [kScanWords * n_scan_tiles]
     DVec<4> *pagg = nullptr;     // [n_scan_tiles] tile aggregates
-    DVec<4> *pinc = nullptr;     // [n_scan_tiles] inclusive tile prefixes
+    DVec<4> *pinc = nullptr;     // [n_scan_tiles] sums of blocks of 64 tiles (at the first)
     int quad = 0;                // 1: quadrupole moments are built and used
     DVec<6> *prefix2 = nullptr;  // [n + 1] exclusive prefix of the second moments
     DVec<6> *pagg2 = nullptr, *pinc2 = nullptr;

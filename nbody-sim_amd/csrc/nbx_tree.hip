@@ -437,7 +437,8 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
                                                  int *__restrict__ vals, int *__restrict__ ghist,
                                                  unsigned *__restrict__ status0, int n_status,
                                                  unsigned long long *__restrict__ scanw,
-                                                 int n_scanw, int *__restrict__ tickets)
+                                                 int n_scanw, int *__restrict__ tickets,
+                                                 int *__restrict__ alloc)
 {
 #pragma clang fp contract(off)
     constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
@@ -458,6 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
     for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
     if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
+    if (gid < 2 * kMaxSeg) alloc[gid] = 0;      // record allocators and depths of the records pass
     __syncthreads();
     const int n = T.n_total;
     float px[kKeysItems], py[kKeysItems], pz[kKeysItems];
@@ -696,7 +698,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
 
 // ===========================================================================
 // Exclusive prefix P[i] = sum_{s<i} e(s) of per-body moment vectors of the
-// sorted bodies in double precision (single pass, decoupled look-back):
+// sorted bodies in double precision (single pass; tile prefixes summed in a
+// fixed order, see below):
 //   KIND 0: e = {m, m x, m y, m z}            (mass and COM of every node)
 //   KIND 1: e = {m xx, m yy, m zz, m xy, m xz, m yz}   (quadrupoles)
 // A node's moments are differences over its range of sorted positions.
@@ -818,51 +821,48 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
         agg = dv_add(agg, s_w[k]);
     }
     if (w == 0) {
-        // publish, then look back over up to 64 earlier tiles of this disk at
-        // once: add aggregates up to the nearest inclusive prefix.  Every
-        // word is stored and loaded as one 64-bit atomic; a tile's value is
-        // complete when none of its N words is still kScanEmpty.
-        if (tile == first_tile) {
-            if (lane == 0) {
-                dv_publish(pinc + tile, agg);
-                s_pre = dv_zero<N>();
-            }
-        } else {
-            if (lane == 0) dv_publish(pagg + tile, agg);
-            DVec<N> pre = dv_zero<N>();
-            int j0 = tile - 1;
-            while (true) {
-                const int j = j0 - lane;
-                DVec<N> ci = dv_zero<N>(), ca = dv_zero<N>();
-                bool inc = true;  // before the disk's first tile: nothing to add
-                if (j >= first_tile) {
-                    while (true) {
-                        if (dv_fetch(pinc + j, ci)) break;
-                        if (dv_fetch(pagg + j, ca)) {
-                            inc = false;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                const uint64_t incm = ballot64(inc);
-                const int stop = incm ? __ffsll((long long)incm) - 1 : 64;  // nearest inclusive
-                DVec<N> c = dv_zero<N>();
-                if (lane < stop) c = ca;
-                else if (lane == stop) c = ci;
+        // Prefix of the tile aggregates with a summation order that never
+        // depends on timing (so a rebuild from the same bodies gives the same
+        // COMs bit for bit).  The tiles of a disk form blocks of 64; the last
+        // tile of a block sums the block's aggregates in a fixed order and
+        // posts the sum at the block's first tile; a tile adds the sums of
+        // the earlier blocks and the aggregates of the earlier tiles of its
+        // own block.  Words are 64-bit relaxed atomics, complete when none is
+        // kScanEmpty; every wait is on a tile with an earlier ticket.
+        const int tl = tile - first_tile, g0 = first_tile + (tl & ~63);
+        const int last_tile = T.stile[s + 1] - 1;
+        auto wait_fetch = [&](const DVec<N> *p) {
+            DVec<N> v;
+            while (!dv_fetch(p, v)) __builtin_amdgcn_s_sleep(1);
+            return v;
+        };
+        auto wave_sum = [&](DVec<N> c) {
 #pragma unroll
-                for (int q = 0; q < N; ++q)
+            for (int q = 0; q < N; ++q)
 #pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) c.v[q] += __shfl_xor(c.v[q], o, 64);
-                pre = dv_add(pre, c);
-                if (incm) break;
-                j0 -= 64;
-            }
-            if (lane == 0) {
-                dv_publish(pinc + tile, dv_add(pre, agg));
-                s_pre = pre;
-            }
+                for (int o = 32; o > 0; o >>= 1) c.v[q] += __shfl_xor(c.v[q], o, 64);
+            return c;
+        };
+        if (lane == 0) dv_publish(pagg + tile, agg);
+        if ((tl & 63) == 63 || tile == last_tile) {  // close this block
+            const int j = g0 + lane;
+            DVec<N> c = dv_zero<N>();
+            if (j < tile) c = wait_fetch(pagg + j);
+            else if (j == tile) c = agg;
+            c = wave_sum(c);
+            if (lane == 0) dv_publish(pinc + g0, c);
         }
+        DVec<N> pre = dv_zero<N>();
+        for (int gb = first_tile; gb < g0; gb += 64 * 64) {  // earlier blocks, 64 at a time
+            const int jb = gb + 64 * lane;
+            DVec<N> c = dv_zero<N>();
+            if (jb < g0) c = wait_fetch(pinc + jb);
+            pre = dv_add(pre, wave_sum(c));
+        }
+        DVec<N> c = dv_zero<N>();  // earlier tiles of this block
+        if (g0 + lane < tile) c = wait_fetch(pagg + g0 + lane);
+        pre = dv_add(pre, wave_sum(c));
+        if (lane == 0) s_pre = pre;
         if (tile == first_tile && lane == 0) P[pbase(T, s)] = dv_zero<N>();
     }
     if (KIND == 0 && tile == 0)  // histograms are consumed: clear them for the next build
@@ -1225,7 +1225,7 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
     hipLaunchKernelGGL(k_keys<BITS>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
                        z, B.partial, B.roots, B.keys[0], B.vals[0], B.ghist, B.status[0],
                        R * B.n_sweep_tiles, (unsigned long long *)B.scanw,
-                       kScanWords * B.n_scan_tiles, B.tickets);
+                       kScanWords * B.n_scan_tiles, B.tickets, B.alloc);
     const int npass = NP + (T.nseg > 1 ? 1 : 0);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
@@ -1265,7 +1265,6 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
 {
     const int ni = T.ioff[T.nseg];
-    launch_fill_int(B.alloc, 0, 2 * kMaxSeg, st);  // alloc and maxlev are adjacent
     if (ni > 0)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
                            B.first, B.alloc, B.maxlev, B.err);

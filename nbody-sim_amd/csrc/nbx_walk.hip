@@ -17,7 +17,7 @@
 
 namespace nbx {
 
-template <int TG, bool QUAD>
+template <int TG, bool QUAD, int U>
 __global__ __launch_bounds__(kWalkBlock) void k_walk(
     SegTable T, WaveMap M, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
     const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     const float4 b = bodies[valid ? pos : T.off[s]];
     GravityVisitor<QUAD> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
     const uint64_t active = ballot64(valid && rr == 0);  // bit tt per valid target
-    wave_walk<TG, QUAD>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
+    wave_walk<TG, QUAD, U>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
     const float fx = slot_sum<TG>(v.fx), fy = slot_sum<TG>(v.fy), fz = slot_sum<TG>(v.fz);
     if (!valid || rr != 0) return;
     const int i = perm[pos];
@@ -316,6 +316,8 @@ __global__ void k_fof_final(int *parent, int *labels, int n)
 // Launchers
 // ===========================================================================
 
+constexpr int kUnroll64 = 2;
+
 template <int TG>
 static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
                            const WalkArgs &W, Bufs &B, hipStream_t st)
@@ -325,7 +327,9 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     for (int s = 0; s < kMaxSeg; ++s)
         M.wave[s + 1] = M.wave[s] + (s < T.nseg ? cdiv(T.off[s + 1] - T.off[s], TG) : 0);
     const int waves = M.wave[T.nseg];
-    auto kern = W.quad ? k_walk<TG, true> : k_walk<TG, false>;
+    // with one target per lane a lane evaluates kUnroll64 records per step
+    constexpr int U = TG >= 16 ? kUnroll64 : 1;
+    auto kern = W.quad ? k_walk<TG, true, U> : k_walk<TG, false, U>;
     hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, M, dtab,
                        step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
                        B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay,

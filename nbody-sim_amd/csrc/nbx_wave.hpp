@@ -87,7 +87,7 @@ struct Rec {
 // owns one target and evaluates every record of the batch.
 // Visitor: visit(rec, leaf) on the lanes that own the record's bit; returns
 // true when the lane opens the (internal) record.
-template <int TG, bool QUAD, class Lds, class Visitor>
+template <int TG, bool QUAD, int U = 1, class Lds, class Visitor>
 __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
                                  const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
                                  int *err, Visitor &vis, int *wstat = nullptr)
@@ -184,36 +184,53 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
                 r.q1 = L.q[2 * k + 1];
             }
         };
-        Rec cur;
-        uint64_t mcur;
-        stage(min(rr, nrec - 1), cur, mcur);
-        for (int k0 = 0; k0 < nrec; k0 += LS) {
-            Rec nxt;
-            uint64_t mnxt;
-            stage(min(k0 + LS + rr, nrec - 1), nxt, mnxt);
-            const bool mine = (k0 + rr < nrec) && ((mcur >> tt) & 1ull);
-            const bool open = vis.visit_masked(cur, mine);
-            const uint64_t om = ballot64(open);
-            if (om) {
+        // a step covers SU = LS * U records: slot rr of unroll u takes record
+        // k0 + u LS + rr, so every lane evaluates U independent records
+        constexpr int SU = LS * U;
+        Rec cur[U];
+        uint64_t mcur[U];
 #pragma unroll
-                for (int q = 0; q < LS; ++q) {
-                    const uint64_t mq = (om >> (q * TG)) & TMASK;
-                    if (mq) {
-                        // every lane of slot q holds record k0 + q
-                        const int first = __builtin_amdgcn_readlane(cur.b.z, q * TG);
-                        const int count = __builtin_amdgcn_readlane(cur.b.w, q * TG);
-                        if (sp < kGStack) {
-                            L.st[sp] = StackEntry{first, count, (unsigned)mq, (unsigned)(mq >> 32)};
-                            ++sp;
-                        } else if (lane == 0) {
-                            atomicOr(err, kErrStack);
+        for (int u = 0; u < U; ++u) stage(min(u * LS + rr, nrec - 1), cur[u], mcur[u]);
+        for (int k0 = 0; k0 < nrec; k0 += SU) {
+            Rec nxt[U];
+            uint64_t mnxt[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                stage(min(k0 + SU + u * LS + rr, nrec - 1), nxt[u], mnxt[u]);
+            bool open[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool mine = (k0 + u * LS + rr < nrec) && ((mcur[u] >> tt) & 1ull);
+                open[u] = vis.visit_masked(cur[u], mine);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t om = ballot64(open[u]);
+                if (om) {
+#pragma unroll
+                    for (int q = 0; q < LS; ++q) {
+                        const uint64_t mq = (om >> (q * TG)) & TMASK;
+                        if (mq) {
+                            // every lane of slot q holds record k0 + u LS + q
+                            const int first = __builtin_amdgcn_readlane(cur[u].b.z, q * TG);
+                            const int count = __builtin_amdgcn_readlane(cur[u].b.w, q * TG);
+                            if (sp < kGStack) {
+                                L.st[sp] = StackEntry{first, count, (unsigned)mq,
+                                                      (unsigned)(mq >> 32)};
+                                ++sp;
+                            } else if (lane == 0) {
+                                atomicOr(err, kErrStack);
+                            }
                         }
                     }
+                    sp_max = max(sp_max, sp);
                 }
-                sp_max = max(sp_max, sp);
             }
-            cur = nxt;
-            mcur = mnxt;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                cur[u] = nxt[u];
+                mcur[u] = mnxt[u];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

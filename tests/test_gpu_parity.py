@@ -520,3 +520,26 @@ def test_walk_autotuner(gpu):
     after = p.numpy()
     for k in before:
         np.testing.assert_array_equal(before[k], after[k])
+
+
+def test_rebuild_is_bit_reproducible(gpu):
+    """Building the tree again from the same bodies gives the same forces bit
+    for bit: every FP64 sum of the build (the moment prefix) has a summation
+    order that does not depend on the timing of the tiles."""
+    import torch
+    nbx = gpu
+    n = 131072
+    d = disk(n, mass=3e-5, seed=77)
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(n)
+    out = []
+    for rep in range(6):
+        print(f"rebuild {rep}", flush=True)
+        nbx.launchBBoxKernel(t, p)
+        nbx.launchBuildKernel(t, p)
+        nbx.launchSummarizeKernel(t)
+        nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0)
+        torch.cuda.synchronize()
+        out.append(torch.stack([p.ax, p.ay, p.az]).cpu().numpy())
+    for o in out[1:]:
+        assert np.array_equal(o, out[0])
