@@ -171,8 +171,11 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
              const nbx_disk_params *dp, int32_t step0, int32_t n_steps);
 
 /* ---- walk shape and autotuner (no reference counterpart) ----------------- */
-/* Digit width of the radix passes of the key sort: 8, 9 or 11 bits (8, 7 or
- * 6 passes over the 63-bit keys).  The sorted order does not depend on it. */
+/* Key sort algorithm: 0 = splitter buckets (sampled splitters, per-bucket
+ * sort in LDS; the default for disks of at most 2^20 bodies), or the digit
+ * width of the radix passes: 8, 9 or 11 bits (8, 7 or 6 passes over the
+ * 63-bit keys; the default above 2^20 bodies).  The sorted order (ascending
+ * key, ties by body index) does not depend on it. */
 int nbx_tree_set_radix_bits(nbx_tree *t, int bits);
 /* Targets per wavefront of the gravity walk (64, 32, 16 or 8; the 64 lanes
  * of a wave split each batch of tree records over 64/targets slots).  Results

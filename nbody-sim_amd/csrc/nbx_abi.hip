@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -128,6 +129,9 @@ int alloc_bufs(nbx_tree *t)
     Bufs &B = t->B;
     const int n = t->T.n_total;
     B.n = n;
+    B.radix_bits = 0;
+    for (int s = 0; s < t->T.nseg; ++s)
+        if (t->T.off[s + 1] - t->T.off[s] > kSplitMaxN) B.radix_bits = 8;
     B.n_sweep_tiles = (n + kSweepTile - 1) / kSweepTile;
     B.n_scan_tiles = t->T.stile[t->T.nseg];
     int rc = NBX_OK;
@@ -138,6 +142,9 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.vals[0], n);
     ALLOC(B.vals[1], n);
     ALLOC(B.ghist, kHistInts);
+    ALLOC(B.spk, t->T.bkoff[t->T.nseg]);
+    ALLOC(B.spi, t->T.bkoff[t->T.nseg]);
+    ALLOC(B.bm, t->T.moff[t->T.nseg]);
     ALLOC(B.status[0], 2048 * B.n_sweep_tiles);
     ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
     ALLOC(B.tickets, 32);
@@ -193,7 +200,8 @@ void free_bufs(nbx_tree *t)
                     B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels,
                     B.step_ctr, B.dtab,    B.prefix2,
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
-                    B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p};
+                    B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
+                    B.bm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
@@ -242,8 +250,14 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
     T.off[nseg] = (int)total;
     T.n_total = (int)total;
     T.blk[0] = T.wave[0] = T.ioff[0] = T.noff[0] = T.stile[0] = 0;
+    T.bkoff[0] = T.sblk[0] = T.moff[0] = 0;
     for (int s = 0; s < nseg; ++s) {
         const int n = counts[s];
+        const int nb = std::min(kMaxBuckets, (n + kBucketMean - 1) / kBucketMean);
+        const int nblk = (n + kSplitTile - 1) / kSplitTile;
+        T.bkoff[s + 1] = T.bkoff[s] + nb;
+        T.sblk[s + 1] = T.sblk[s] + nblk;
+        T.moff[s + 1] = T.moff[s] + nb * nblk;
         T.stile[s + 1] = T.stile[s] + (n + kScanTile - 1) / kScanTile;
         T.blk[s + 1] = T.blk[s] + (n + kBlock - 1) / kBlock;
         T.wave[s + 1] = T.wave[s] + (n + kWave - 1) / kWave;
@@ -257,6 +271,9 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
         T.ioff[s] = T.ioff[nseg];
         T.noff[s] = T.noff[nseg];
         T.stile[s] = T.stile[nseg];
+        T.bkoff[s] = T.bkoff[nseg];
+        T.sblk[s] = T.sblk[nseg];
+        T.moff[s] = T.moff[nseg];
     }
     hipError_t e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
@@ -521,7 +538,12 @@ int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float 
 int nbx_tree_set_radix_bits(nbx_tree *t, int bits)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
-    if (bits != 8 && bits != 9 && bits != 11) return fail(NBX_E_ARG, "radix bits %d not in {8, 9, 11}", bits);
+    if (bits != 0 && bits != 8 && bits != 9 && bits != 11)
+        return fail(NBX_E_ARG, "radix bits %d not in {0, 8, 9, 11}", bits);
+    if (bits == 0)
+        for (int s = 0; s < t->T.nseg; ++s)
+            if (t->T.off[s + 1] - t->T.off[s] > kSplitMaxN)
+                return fail(NBX_E_ARG, "splitter sort takes disks of at most %d bodies", kSplitMaxN);
     t->B.radix_bits = bits;
     t->stage = kNone;
     if (t->graph) {
@@ -862,7 +884,7 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
         launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
                                p->az, dth, dt, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 1);
-    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false);
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);
     // The collision pass reads the tree and the velocities and writes only

@@ -25,6 +25,17 @@ constexpr int kWave = 64;        // gfx950 wavefront
 constexpr int kStackCap = 384;   // gravity-walk stack entries per wave (LDS)
 constexpr int kRecPad = 16;      // records of padding after the node array
 
+// Splitter-bucket key sort (disks of at most kSplitMaxN bodies).
+constexpr int kSplitItems = 4;                     // bodies per thread of a scatter block
+constexpr int kSplitTile = kBlock * kSplitItems;   // bodies per scatter block
+constexpr int kBucketMean = 512;                   // target bodies per bucket
+constexpr int kMaxBuckets = 1024;                  // buckets per disk
+constexpr int kOversample = 8;                     // samples per bucket
+constexpr int kMaxSample = kOversample * kMaxBuckets;
+constexpr int kBucketCap = 4096;                   // largest bucket sorted in LDS
+constexpr int kSplitMaxN = 1 << 20;                // larger disks use the radix passes
+constexpr int kMaxSplitBlocks = kSplitMaxN / kSplitTile;
+
 // Per-disk offsets, passed by value to every kernel.
 struct SegTable {
     int nseg;
@@ -35,6 +46,9 @@ struct SegTable {
     int ioff[kMaxSeg + 1];       // binary internal-node offset (n_s - 1 each)
     int noff[kMaxSeg + 1];       // walk-record offset (2 n_s + 1 each)
     int stile[kMaxSeg + 1];      // prefix-scan tile offset (tiles never straddle disks)
+    int bkoff[kMaxSeg + 1];      // sort-bucket offset (kBucketMean bodies per bucket)
+    int sblk[kMaxSeg + 1];       // scatter-block offset (kSplitTile bodies per block)
+    int moff[kMaxSeg + 1];       // offset of the disk's (bucket x scatter block) count matrix
 };
 
 __host__ __device__ inline int seg_of_block(const SegTable &T, int b)
@@ -59,6 +73,18 @@ __host__ __device__ inline int seg_of_stile(const SegTable &T, int t)
 {
     int s = 0;
     while (s + 1 < T.nseg && t >= T.stile[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_bucket(const SegTable &T, int g)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && g >= T.bkoff[s + 1]) ++s;
+    return s;
+}
+__host__ __device__ inline int seg_of_sblock(const SegTable &T, int b)
+{
+    int s = 0;
+    while (s + 1 < T.nseg && b >= T.sblk[s + 1]) ++s;
     return s;
 }
 __host__ __device__ inline int seg_of_internal(const SegTable &T, int j)

@@ -20,7 +20,11 @@ struct Bufs {
     uint64_t *keys[2] = {};      // double-buffered sort keys
     int *vals[2] = {};           // double-buffered body indices
     int *ghist = nullptr;        // [kHistInts] global digit histograms of all passes
-    int radix_bits = 8;          // digit width of the radix passes: 8, 9 or 11
+    int radix_bits = 0;          // 0: splitter-bucket sort; 8, 9 or 11: radix digit width
+    uint64_t *spk = nullptr;     // [buckets] lower splitter key of every bucket (first unused)
+    int *spi = nullptr;          // [buckets] body index of that splitter
+    int *bm = nullptr;           // [moff] packed {offset << 16 | count} per (bucket, scatter block)
+    bool split_ready = false;    // spk / spi hold the quantiles of the last splitter sort
     unsigned *status[2] = {};    // [n_sweep_tiles * 2048] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
     DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
@@ -125,8 +129,10 @@ void launch_diagnostics(const float *x, const float *y, const float *z, const fl
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
                  hipStream_t st);
 void launch_root(const SegTable &T, Bufs &B, hipStream_t st);
+// resample = false lets the splitter sort reuse the splitters of the previous
+// build (the step loop); every other build samples them afresh.
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st);
+                  const float *mass, Bufs &B, hipStream_t st, bool resample = true);
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st);
 void launch_fill_int(int *p, int v, int n, hipStream_t st);
 void launch_fill_float(float *p, float v, int n, hipStream_t st);

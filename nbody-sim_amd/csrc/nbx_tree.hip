@@ -418,14 +418,53 @@ __global__ __launch_bounds__(kBlock) void k_bbox(SegTable T, const float *__rest
 
 // ===========================================================================
 // Keys: the octant digits of BHTree.cu:164-170 along the float descent of
-// BHTree.cu:253-256, 21 levels, first level most significant.  The same
-// pass accumulates the digit histograms of all radix passes (BITS-bit
-// digits) and clears the look-back words of the first pass and of the
-// prefix scans.  Each block covers kKeysPerBlock bodies (8 per thread) so
-// that few blocks post histograms.
+// BHTree.cu:253-256, 21 levels, first level most significant.  The key pass
+// also reduces the bounding-box partials to the root cubes and clears the
+// look-back words and counters of the passes that follow.  For the radix
+// sort it accumulates the digit histograms of all passes (each block covers
+// kKeysPerBlock bodies, 8 per thread, so that few blocks post histograms);
+// for the splitter sort it also performs the bucket scatter (below).
 // ===========================================================================
 constexpr int kKeysItems = 8;
 constexpr int kKeysPerBlock = kBlock * kKeysItems;
+
+__device__ __forceinline__ uint64_t morton_key(float px, float py, float pz, const Root &R0)
+{
+#pragma clang fp contract(off)
+    float cx = R0.c[0], cy = R0.c[1], cz = R0.c[2], hh = R0.c[3];
+    uint64_t key = 0;
+#pragma unroll
+    for (int l = 0; l < kLevels; ++l) {
+        const bool ox = px > cx, oy = py > cy, oz = pz > cz;
+        key = (key << 3) | (uint64_t)((ox ? 1 : 0) | (oy ? 2 : 0) | (oz ? 4 : 0));
+        hh = hh * 0.5f;
+        cx = cx + (ox ? hh : -hh);
+        cy = cy + (oy ? hh : -hh);
+        cz = cz + (oz ? hh : -hh);
+    }
+    return key;
+}
+
+// Root cubes of disks [s0, s1] into s_root (block 0: of every disk, also
+// published to roots[]), and the resets owed by the key pass.
+__device__ void keys_prologue(const SegTable &T, const float *__restrict__ partial,
+                              Root *__restrict__ roots, Root *s_root, int s0, int s1,
+                              unsigned *__restrict__ status0, int n_status,
+                              unsigned long long *__restrict__ scanw, int n_scanw,
+                              int *__restrict__ tickets, int *__restrict__ alloc)
+{
+    if (blockIdx.x == 0) {
+        s0 = 0;
+        s1 = T.nseg - 1;
+    }
+    for (int sg = s0; sg <= s1; ++sg) root_of_partials(T, partial, sg, &s_root[sg]);
+    if (blockIdx.x == 0 && threadIdx.x < T.nseg) roots[threadIdx.x] = s_root[threadIdx.x];
+    const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
+    for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
+    for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
+    if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
+    if (gid < 2 * kMaxSeg) alloc[gid] = 0;      // record allocators and depths of the records pass
+}
 
 template <int BITS>
 __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__restrict__ x,
@@ -440,28 +479,19 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
                                                  int n_scanw, int *__restrict__ tickets,
                                                  int *__restrict__ alloc)
 {
-#pragma clang fp contract(off)
     constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
     __shared__ int h[NP * R];
     __shared__ Root s_root[kMaxSeg];
-    for (int q = threadIdx.x; q < NP * R; q += kBlock) h[q] = 0;
-    {
-        // root cubes of the disks this block's bodies belong to (block 0:
-        // of every disk, published for the later passes)
-        const int b0 = blockIdx.x * kKeysPerBlock;
-        const int b1 = min(T.n_total, b0 + kKeysPerBlock) - 1;
-        const int s0 = blockIdx.x == 0 ? 0 : seg_of_body(T, b0);
-        const int s1 = blockIdx.x == 0 ? T.nseg - 1 : seg_of_body(T, b1);
-        for (int sg = s0; sg <= s1; ++sg) root_of_partials(T, partial, sg, &s_root[sg]);
-        if (blockIdx.x == 0 && threadIdx.x < T.nseg) roots[threadIdx.x] = s_root[threadIdx.x];
-    }
-    const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
-    for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
-    for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
-    if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
-    if (gid < 2 * kMaxSeg) alloc[gid] = 0;      // record allocators and depths of the records pass
-    __syncthreads();
+    if (ghist)
+        for (int q = threadIdx.x; q < NP * R; q += kBlock) h[q] = 0;
     const int n = T.n_total;
+    {
+        const int b0 = blockIdx.x * kKeysPerBlock;
+        const int b1 = min(n, b0 + kKeysPerBlock) - 1;
+        keys_prologue(T, partial, roots, s_root, seg_of_body(T, b0), seg_of_body(T, b1),
+                      status0, n_status, scanw, n_scanw, tickets, alloc);
+    }
+    __syncthreads();
     float px[kKeysItems], py[kKeysItems], pz[kKeysItems];
 #pragma unroll
     for (int r = 0; r < kKeysItems; ++r) {  // all loads first: one memory latency
@@ -474,24 +504,16 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
     for (int r = 0; r < kKeysItems; ++r) {
         const int i = blockIdx.x * kKeysPerBlock + r * kBlock + threadIdx.x;
         if (i >= n) break;
-        const Root R0 = s_root[seg_of_body(T, i)];
-        float cx = R0.c[0], cy = R0.c[1], cz = R0.c[2], hh = R0.c[3];
-        uint64_t key = 0;
-#pragma unroll
-        for (int l = 0; l < kLevels; ++l) {
-            const bool ox = px[r] > cx, oy = py[r] > cy, oz = pz[r] > cz;
-            key = (key << 3) | (uint64_t)((ox ? 1 : 0) | (oy ? 2 : 0) | (oz ? 4 : 0));
-            hh = hh * 0.5f;
-            cx = cx + (ox ? hh : -hh);
-            cy = cy + (oy ? hh : -hh);
-            cz = cz + (oz ? hh : -hh);
-        }
+        const uint64_t key = morton_key(px[r], py[r], pz[r], s_root[seg_of_body(T, i)]);
         keys[i] = key;
-        vals[i] = i;
+        if (ghist) {  // radix passes: index values and digit histograms
+            vals[i] = i;
 #pragma unroll
-        for (int p = 0; p < NP; ++p)
-            atomicAdd(&h[p * R + (int)((key >> (BITS * p)) & (uint64_t)(R - 1))], 1);
+            for (int p = 0; p < NP; ++p)
+                atomicAdd(&h[p * R + (int)((key >> (BITS * p)) & (uint64_t)(R - 1))], 1);
+        }
     }
+    if (!ghist) return;
     __syncthreads();
     for (int q = threadIdx.x; q < NP * R; q += kBlock)
         if (h[q]) atomicAdd(&ghist[q], h[q]);
@@ -695,6 +717,421 @@ __global__ __launch_bounds__(kBlock) void k_sweep(
         }
     }
 }
+
+// ===========================================================================
+// Splitter-bucket sort.  The sorted order is the ascending order of the
+// pairs (key, body index): the order the stable radix passes above produce
+// from index-ordered input, so either sort feeds the same tree.
+//   splitters      bucket b of a disk holds the pairs from its lower
+//                  splitter up to the next one.  Every bucket sort writes,
+//                  as its splitters for the next build, the pairs at the
+//                  quantile positions floor(b n / buckets) of the order it
+//                  produced (bodies move little between steps, so buckets
+//                  stay near kBucketMean members).  A build without them
+//                  (the first of a tree, and every build outside the step
+//                  loop) samples kOversample keys per bucket (k_split).
+//   scatter        blocks of kSplitTile bodies of one disk find each body's
+//                  bucket by binary search over the splitters in LDS, group
+//                  the block's bodies by bucket (LDS counters and a block
+//                  scan) and store them at the block's own body range, with
+//                  one packed {offset << 16 | count} word per (bucket,
+//                  block).  No global atomics.  In the step loop the scatter
+//                  runs inside the key pass (k_keys_scatter).
+//   k_bucket_sort  one block per bucket gathers the bucket's members through
+//                  its column of words, sorts them with an all-ascending
+//                  bitonic network in LDS (a bucket beyond kBucketCap
+//                  members, possible only when the splitters misrepresent
+//                  the keys, is sorted in place in global memory) and writes
+//                  keys, indices and the gathered {x, y, z, m} at the
+//                  bucket's offset: the sum of the in-block offsets of its
+//                  words.
+// Every (key, index) pair is distinct, so the result depends neither on the
+// splitters nor on the order in which LDS atomics rank a bucket's members.
+// ===========================================================================
+__device__ __forceinline__ bool kv_less(uint64_t ka, int ia, uint64_t kb, int ib)
+{
+    return ka < kb || (ka == kb && ia < ib);
+}
+
+constexpr uint64_t kPadKey = ~0ull;  // above every 63-bit key
+constexpr int kPadIdx = 0x7fffffff;
+
+// Exclusive scan over the NT threads of a block; *total receives the sum.
+template <int NT>
+__device__ inline int block_excl_scan(int v, int *total)
+{
+    __shared__ int s_w[NT / kWave];
+    const int lane = __lane_id(), w = threadIdx.x / kWave;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int y = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += y;
+    }
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NT / kWave; ++k) {
+        const int c = s_w[k];
+        if (k < w) before += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + inc - v;
+}
+
+// One comparator stage of the all-ascending bitonic network on one element
+// per lane: partners are lane and lane ^ mask, hib is the top bit of mask,
+// and the lower lane of every pair keeps the smaller element.  Elements are
+// distinct except padding, and equal elements may be exchanged freely, so
+// the upper lane takes the partner exactly when the lower one does not.
+__device__ __forceinline__ void wave_cmpx(uint64_t &k, int &id, int lane, int mask, int hib)
+{
+    const unsigned plo = (unsigned)__shfl_xor((int)(unsigned)k, mask, kWave);
+    const unsigned phi = (unsigned)__shfl_xor((int)(unsigned)(k >> 32), mask, kWave);
+    const int pid = __shfl_xor(id, mask, kWave);
+    const uint64_t pk = ((uint64_t)phi << 32) | plo;
+    if (kv_less(pk, pid, k, id) == ((lane & hib) == 0)) {
+        k = pk;
+        id = pid;
+    }
+}
+
+template <class KP, class IP>
+__device__ __forceinline__ void cmpx_mem(KP sk, IP si, int i, int l)
+{
+    const uint64_t a = sk[i], b = sk[l];
+    const int ia = si[i], ib = si[l];
+    if (kv_less(b, ib, a, ia)) {
+        sk[i] = b;
+        si[i] = ib;
+        sk[l] = a;
+        si[l] = ia;
+    }
+}
+
+// Sorts sk/si[0, c) ascending by (key, index) with all NT threads of the
+// block.  The network is the bitonic sorter in its all-ascending form (each
+// merge starts with the mirrored comparator i <-> i ^ (k - 1)), so the
+// virtual padding of [c, 2^ceil(log2 c)) with +inf never moves and every
+// comparator that touches it is skipped.  Stages within 64 elements run in
+// registers (lane shuffles).  sk/si are the block's own LDS or global
+// memory (separate instantiations, so that LDS stays on ds_* accesses).
+template <int NT, class KP, class IP>
+__device__ __forceinline__ void block_sort(KP sk, IP si, int c)
+{
+    constexpr int NW = NT / kWave;
+    const int lane = __lane_id(), w = threadIdx.x / kWave;
+    const int nch = (c + kWave - 1) / kWave;
+    for (int ch = w; ch < nch; ch += NW) {
+        const int i = ch * kWave + lane;
+        uint64_t k = i < c ? sk[i] : kPadKey;
+        int id = i < c ? si[i] : kPadIdx;
+#pragma unroll
+        for (int kk = 2; kk <= kWave; kk <<= 1) {
+            wave_cmpx(k, id, lane, kk - 1, kk >> 1);
+#pragma unroll
+            for (int j = kk >> 2; j > 0; j >>= 1) wave_cmpx(k, id, lane, j, j);
+        }
+        if (i < c) {
+            sk[i] = k;
+            si[i] = id;
+        }
+    }
+    __syncthreads();
+    int P = kWave;
+    while (P < c) P <<= 1;
+    for (int kk = 2 * kWave; kk <= P; kk <<= 1) {
+        const int half = kk >> 1;
+        for (int q = threadIdx.x; q < P / 2; q += NT) {
+            const int i = q + (q & ~(half - 1));  // lower half of its kk-block
+            const int l = i ^ (kk - 1);
+            if (l < c) cmpx_mem(sk, si, i, l);
+        }
+        __syncthreads();
+        for (int j = kk >> 2; j >= kWave; j >>= 1) {
+            for (int q = threadIdx.x; q < P / 2; q += NT) {
+                const int i = q + (q & ~(j - 1));
+                const int l = i + j;
+                if (l < c) cmpx_mem(sk, si, i, l);
+            }
+            __syncthreads();
+        }
+        for (int ch = w; ch < nch; ch += NW) {
+            const int i = ch * kWave + lane;
+            uint64_t k = i < c ? sk[i] : kPadKey;
+            int id = i < c ? si[i] : kPadIdx;
+#pragma unroll
+            for (int j = kWave / 2; j > 0; j >>= 1) wave_cmpx(k, id, lane, j, j);
+            if (i < c) {
+                sk[i] = k;
+                si[i] = id;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kSplitThreads = 1024;
+
+__global__ __launch_bounds__(kSplitThreads) void k_split(SegTable T,
+                                                         const uint64_t *__restrict__ keys,
+                                                         uint64_t *__restrict__ spk,
+                                                         int *__restrict__ spi)
+{
+    __shared__ uint64_t sk[kMaxSample];
+    __shared__ int si[kMaxSample];
+    const int s = blockIdx.x;
+    const int n = T.off[s + 1] - T.off[s];
+    const int nb = T.bkoff[s + 1] - T.bkoff[s];
+    if (nb <= 1) return;
+    const int S = min(n, kOversample * nb);
+    const double stride = (double)n / (double)S;  // >= 1: distinct indices
+    for (int j = threadIdx.x; j < S; j += kSplitThreads) {
+        const int idx = T.off[s] + (int)(((double)j + 0.5) * stride);
+        sk[j] = keys[idx];
+        si[j] = idx;
+    }
+    __syncthreads();
+    block_sort<kSplitThreads>(sk, si, S);
+    for (int b = 1 + threadIdx.x; b < nb; b += kSplitThreads) {
+        const int q = (int)((long long)b * S / nb);
+        spk[T.bkoff[s] + b] = sk[q];
+        spi[T.bkoff[s] + b] = si[q];
+    }
+}
+
+// Scatter of one block of kSplitTile bodies of disk s (block kb of the
+// disk): kk[r] is the key of body base + r kBlock + threadIdx.x (valid below
+// end).  All threads of the block call it.
+__device__ __forceinline__ void scatter_tile(const SegTable &T, int s, int kb, int base, int end,
+                                             const uint64_t (&kk)[kSplitItems],
+                                             const uint64_t *__restrict__ spk,
+                                             const int *__restrict__ spi,
+                                             uint64_t *__restrict__ skey,
+                                             int *__restrict__ sidx, int *__restrict__ bm)
+{
+    __shared__ uint64_t s_k[kMaxBuckets];
+    __shared__ int s_i[kMaxBuckets];
+    __shared__ int s_c[kMaxBuckets];
+    __shared__ int s_o[kMaxBuckets];
+    constexpr int BPT = kMaxBuckets / kBlock;  // buckets per thread in the count scan
+    const int t = threadIdx.x;
+    const int nb = T.bkoff[s + 1] - T.bkoff[s];
+    const int nblk = T.sblk[s + 1] - T.sblk[s];
+    for (int q = t; q < nb; q += kBlock) {
+        s_c[q] = 0;
+        if (q > 0) {
+            s_k[q] = spk[T.bkoff[s] + q];
+            s_i[q] = spi[T.bkoff[s] + q];
+        }
+    }
+    __syncthreads();
+    int top = 1;
+    while (2 * top < nb) top <<= 1;
+    int bk[kSplitItems], lr[kSplitItems];
+#pragma unroll
+    for (int r = 0; r < kSplitItems; ++r) {
+        const int i = base + r * kBlock + t;
+        bk[r] = -1;
+        lr[r] = 0;
+        if (i >= end) continue;
+        // largest bucket whose lower splitter is <= (key, i); bucket 0 has none
+        int lo = 0;
+        for (int step = top; step > 0; step >>= 1) {
+            const int cand = lo + step;
+            if (cand < nb && !kv_less(kk[r], i, s_k[cand], s_i[cand])) lo = cand;
+        }
+        bk[r] = lo;
+        lr[r] = atomicAdd(&s_c[lo], 1);
+    }
+    __syncthreads();
+    // exclusive offsets of the buckets inside the block
+    int c[BPT], sum = 0;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int q = t * BPT + u;
+        c[u] = q < nb ? s_c[q] : 0;
+        sum += c[u];
+    }
+    int total;
+    int run = block_excl_scan<kBlock>(sum, &total);
+    int *col = bm + T.moff[s] + kb;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        const int q = t * BPT + u;
+        if (q < nb) {
+            s_o[q] = run;
+            col[q * nblk] = (run << 16) | c[u];
+        }
+        run += c[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSplitItems; ++r) {
+        if (bk[r] < 0) continue;
+        const int pos = base + s_o[bk[r]] + lr[r];
+        skey[pos] = kk[r];
+        sidx[pos] = base + r * kBlock + t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(SegTable T, const uint64_t *__restrict__ keys,
+                                                    const uint64_t *__restrict__ spk,
+                                                    const int *__restrict__ spi,
+                                                    uint64_t *__restrict__ skey,
+                                                    int *__restrict__ sidx, int *__restrict__ bm)
+{
+    const int s = seg_of_sblock(T, blockIdx.x);
+    const int kb = blockIdx.x - T.sblk[s];
+    const int base = T.off[s] + kb * kSplitTile;
+    const int end = min(T.off[s + 1], base + kSplitTile);
+    uint64_t kk[kSplitItems];
+#pragma unroll
+    for (int r = 0; r < kSplitItems; ++r) {
+        const int i = base + r * kBlock + threadIdx.x;
+        kk[r] = i < end ? keys[i] : 0ull;
+    }
+    scatter_tile(T, s, kb, base, end, kk, spk, spi, skey, sidx, bm);
+}
+
+// Key pass and scatter in one kernel (splitters of the previous build).
+__global__ __launch_bounds__(kBlock) void k_keys_scatter(
+    SegTable T, const float *__restrict__ x, const float *__restrict__ y,
+    const float *__restrict__ z, const float *__restrict__ partial, Root *__restrict__ roots,
+    unsigned long long *__restrict__ scanw, int n_scanw, int *__restrict__ tickets,
+    int *__restrict__ alloc, const uint64_t *__restrict__ spk, const int *__restrict__ spi,
+    uint64_t *__restrict__ skey, int *__restrict__ sidx, int *__restrict__ bm)
+{
+    __shared__ Root s_root[kMaxSeg];
+    const int s = seg_of_sblock(T, blockIdx.x);
+    const int kb = blockIdx.x - T.sblk[s];
+    const int base = T.off[s] + kb * kSplitTile;
+    const int end = min(T.off[s + 1], base + kSplitTile);
+    float px[kSplitItems], py[kSplitItems], pz[kSplitItems];
+#pragma unroll
+    for (int r = 0; r < kSplitItems; ++r) {
+        const int i = base + r * kBlock + threadIdx.x;
+        px[r] = i < end ? x[i] : 0.f;
+        py[r] = i < end ? y[i] : 0.f;
+        pz[r] = i < end ? z[i] : 0.f;
+    }
+    keys_prologue(T, partial, roots, s_root, s, s, nullptr, 0, scanw, n_scanw, tickets, alloc);
+    __syncthreads();
+    uint64_t kk[kSplitItems];
+#pragma unroll
+    for (int r = 0; r < kSplitItems; ++r) kk[r] = morton_key(px[r], py[r], pz[r], s_root[s]);
+    scatter_tile(T, s, kb, base, end, kk, spk, spi, skey, sidx, bm);
+}
+
+constexpr int kSortThreads = 512;
+
+template <int NT, class KP, class IP>
+__device__ __forceinline__ void gather_bucket(KP dk, IP di, const uint64_t *__restrict__ skey,
+                                              const int *__restrict__ sidx, const int *s_pre,
+                                              const int *s_src, int nblk, int c)
+{
+    int top = 1;
+    while (2 * top <= nblk) top <<= 1;
+    for (int j = threadIdx.x; j < c; j += NT) {
+        // contribution k with s_pre[k] <= j < s_pre[k + 1]
+        int k = 0;
+        for (int step = top; step > 0; step >>= 1)
+            if (k + step < nblk && s_pre[k + step] <= j) k += step;
+        const int q = s_src[k] + (j - s_pre[k]);
+        dk[j] = skey[q];
+        di[j] = sidx[q];
+    }
+}
+
+// Gathered bodies of a sorted bucket, and the splitters of the next build:
+// the pairs at the quantile positions floor(b n / nb) of the disk.
+template <int NT, class KP, class IP>
+__device__ __forceinline__ void bucket_emit(const SegTable &T, int s, KP dk, IP di, int c,
+                                            int earlier, const float *__restrict__ x,
+                                            const float *__restrict__ y,
+                                            const float *__restrict__ z,
+                                            const float *__restrict__ m,
+                                            float4 *__restrict__ bodies,
+                                            uint64_t *__restrict__ spk, int *__restrict__ spi)
+{
+    const int n = T.off[s + 1] - T.off[s];
+    const int nb = T.bkoff[s + 1] - T.bkoff[s];
+    for (int j = threadIdx.x; j < c; j += NT) {
+        const int v = di[j];
+        bodies[j] = make_float4(x[v], y[v], z[v], m[v]);
+        // p nb and q n stay below 2^31 (n <= kSplitMaxN, nb <= kMaxBuckets)
+        const int p = earlier + j;  // position inside the disk
+        const int q = (p * nb + n - 1) / n;
+        if (q >= 1 && q < nb && q * n / nb == p) {
+            spk[T.bkoff[s] + q] = dk[j];
+            spi[T.bkoff[s] + q] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_bucket_sort(
+    SegTable T, const uint64_t *__restrict__ skey, const int *__restrict__ sidx,
+    const int *__restrict__ bm, uint64_t *__restrict__ kout, int *__restrict__ vout,
+    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
+    const float *__restrict__ m, float4 *__restrict__ bodies, uint64_t *__restrict__ spk,
+    int *__restrict__ spi)
+{
+    __shared__ uint64_t sk[kBucketCap];
+    __shared__ int si[kBucketCap];
+    __shared__ int s_pre[kMaxSplitBlocks + 1];  // first bucket member of each contribution
+    __shared__ int s_src[kMaxSplitBlocks];      // staging position of each contribution
+    constexpr int CPT = kMaxSplitBlocks / kSortThreads;
+    const int t = threadIdx.x;
+    const int g = blockIdx.x;
+    const int s = seg_of_bucket(T, g);
+    const int b = g - T.bkoff[s];
+    const int nblk = T.sblk[s + 1] - T.sblk[s];
+    const int *row = bm + T.moff[s] + b * nblk;
+    int cw[CPT], csum = 0, osum = 0;
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int k = t * CPT + u;
+        cw[u] = k < nblk ? row[k] : 0;
+        csum += cw[u] & 0xffff;
+        osum += cw[u] >> 16;
+    }
+    int c, earlier;
+    int run = block_excl_scan<kSortThreads>(csum, &c);
+    // members of disk s in earlier buckets: the sum over blocks of the offsets
+    (void)block_excl_scan<kSortThreads>(osum, &earlier);
+#pragma unroll
+    for (int u = 0; u < CPT; ++u) {
+        const int k = t * CPT + u;
+        if (k < nblk) {
+            s_pre[k] = run;
+            s_src[k] = T.off[s] + k * kSplitTile + (cw[u] >> 16);
+        }
+        run += cw[u] & 0xffff;
+    }
+    __syncthreads();
+    const int out = T.off[s] + earlier;
+    uint64_t *ok = kout + out;
+    int *ov = vout + out;
+    if (c <= kBucketCap) {
+        gather_bucket<kSortThreads>(sk, si, skey, sidx, s_pre, s_src, nblk, c);
+        __syncthreads();
+        block_sort<kSortThreads>(sk, si, c);
+        for (int j = t; j < c; j += kSortThreads) {
+            ok[j] = sk[j];
+            ov[j] = si[j];
+        }
+        bucket_emit<kSortThreads>(T, s, sk, si, c, earlier, x, y, z, m, bodies + out, spk, spi);
+    } else {  // in place at the bucket's final range
+        gather_bucket<kSortThreads>(ok, ov, skey, sidx, s_pre, s_src, nblk, c);
+        __syncthreads();
+        block_sort<kSortThreads>(ok, ov, c);
+        bucket_emit<kSortThreads>(T, s, ok, ov, c, earlier, x, y, z, m, bodies + out, spk, spi);
+    }
+}
+
 
 // ===========================================================================
 // Exclusive prefix P[i] = sum_{s<i} e(s) of per-body moment vectors of the
@@ -1240,11 +1677,38 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
     B.cur = cur;
 }
 
+static void launch_split_sort(const SegTable &T, const float *x, const float *y, const float *z,
+                              const float *mass, Bufs &B, bool resample, hipStream_t st)
+{
+    const int n = T.n_total;
+    if (resample || !B.split_ready) {
+        hipLaunchKernelGGL(k_keys<8>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
+                           z, B.partial, B.roots, B.keys[0], nullptr, nullptr, nullptr, 0,
+                           (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
+                           B.alloc);
+        hipLaunchKernelGGL(k_split, dim3(T.nseg), dim3(kSplitThreads), 0, st, T, B.keys[0],
+                           B.spk, B.spi);
+        hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0],
+                           B.spk, B.spi, B.keys[1], B.vals[1], B.bm);
+    } else {
+        hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
+                           B.partial, B.roots, (unsigned long long *)B.scanw,
+                           kScanWords * B.n_scan_tiles, B.tickets, B.alloc, B.spk, B.spi,
+                           B.keys[1], B.vals[1], B.bm);
+    }
+    hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads), 0, st, T,
+                       B.keys[1], B.vals[1], B.bm, B.keys[0], B.vals[0], x, y, z, mass, B.bodies,
+                       B.spk, B.spi);
+    B.cur = 0;
+    B.split_ready = true;
+}
+
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st)
+                  const float *mass, Bufs &B, hipStream_t st, bool resample)
 {
     const int n = T.n_total;
     switch (B.radix_bits) {
+    case 0: launch_split_sort(T, x, y, z, mass, B, resample, st); break;
     case 9: launch_sort<9>(T, x, y, z, mass, B, st); break;
     case 11: launch_sort<11>(T, x, y, z, mass, B, st); break;
     default: launch_sort<8>(T, x, y, z, mass, B, st); break;

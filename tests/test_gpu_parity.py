@@ -7,6 +7,7 @@ tolerance stated in each test.
 """
 import numpy as np
 import pytest
+import torch
 
 import pyoracle as O
 from helpers import clustered, disk, rel_err
@@ -85,7 +86,9 @@ def test_perturber_kernel_kats(gpu):
 # Keys and order: bit-exact against the oracle and against the reference DFS
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("n,seed,bits", [(10, 0, 8), (1000, 1, 8), (4096, 2, 8), (131072, 3, 8),
-                                         (131072, 3, 9), (131072, 3, 11), (5000, 4, 11)])
+                                         (131072, 3, 9), (131072, 3, 11), (5000, 4, 11),
+                                         (10, 0, 0), (513, 1, 0), (1000, 1, 0), (4096, 2, 0),
+                                         (5000, 4, 0), (131072, 3, 0), (1 << 20, 6, 0)])
 def test_keys_and_order_bit_exact(gpu, n, seed, bits):
     nbx = gpu
     d = disk(n, seed=seed)
@@ -107,6 +110,79 @@ def test_keys_and_order_bit_exact(gpu, n, seed, bits):
         ref = O.RefTree(d["x"], d["y"], d["z"], d["mass"])
         if len(np.unique(keys)) == n:
             np.testing.assert_array_equal(g_perm, ref.sort_idx())
+
+
+def _order_vs_oracle(nbx, d, bits=None):
+    p = nbx.to_gpu(d)
+    t = nbx.allocateOctree(len(d["x"]))
+    if bits is not None:
+        nbx.set_radix_bits(t, bits)
+    t = _build(nbx, p, t)
+    keys = O.mt_keys(d["x"], d["y"], d["z"])
+    perm = O.mt_sort(keys)
+    np.testing.assert_array_equal(_np(t.sort_idx()), perm)
+    np.testing.assert_array_equal(_np(t.sorted_keys()).view(np.uint64), keys[perm])
+    return keys
+
+
+def test_split_sort_unrepresentative_sample(gpu):
+    """Splitter sort when the sampled bodies (every 64th index from 32 at
+    n = 65536) all sit in one small clump: the splitters then cut only the
+    clump, the rest of the disk lands in two buckets far above the LDS
+    capacity, and those are sorted in place in global memory."""
+    nbx = gpu
+    n = 65536
+    d = disk(n, seed=11)
+    rng = np.random.default_rng(12)
+    sel = np.arange(32, n, 64)
+    for k, c in (("x", 3.0), ("y", 0.01), ("z", 0.0)):
+        d[k][sel] = (c + rng.normal(0, 1e-3, len(sel))).astype(np.float32)
+    print("split sort, clumped sample n=65536", flush=True)
+    keys = _order_vs_oracle(nbx, d)
+    clump = (keys >= keys[sel].min()) & (keys <= keys[sel].max())
+    assert (~clump).sum() > 2 * 4096  # the oversized buckets really occur
+
+
+def test_split_sort_equal_keys(gpu):
+    """Thousands of bodies on one point share all 21 octant digits: the
+    order among them is by body index, as in the stable radix passes."""
+    nbx = gpu
+    d = disk(20000, seed=13)
+    for k in ("x", "y", "z"):
+        d[k][5000:10000] = d[k][7]
+    d["z"][12000:12100] = np.float32(0.0)
+    keys = _order_vs_oracle(nbx, d)
+    assert len(np.unique(keys)) < 20000 - 4000
+    _order_vs_oracle(nbx, d, bits=8)
+
+
+def test_split_sort_steps_equal_radix_steps(gpu):
+    """Ten graph-replayed steps with collisions under either key sort give
+    bit-identical states (the tree, and so every force, is the same)."""
+    nbx = gpu
+    n = 50000
+    d = disk(n, seed=14)
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True, use_graph=True)
+    dp = nbx.disk_params(np.float32(2e-3), nbx.makePerturber(True, 10.0, 5.2, 0.0, 1.0))
+    # then the bodies jump to a clumped layout without a fresh init: the step
+    # loop's splitters (quantiles of the previous build) misfit it badly
+    jump = clustered(50, n // 50, 0.02, seed=16)
+    out = []
+    for bits in (0, 8):
+        p = nbx.to_gpu(d)
+        t = nbx.allocateOctree(n)
+        nbx.set_radix_bits(t, bits)
+        nbx.init_forces(t, p, sp, dp)
+        nbx.step(t, p, sp, dp, 0, 10)
+        out.append(p.numpy())
+        for k in ("x", "y", "z"):
+            getattr(p, k).copy_(torch.from_numpy(jump[k]))
+        nbx.step(t, p, sp, dp, 10, 2)
+        out.append(p.numpy())
+    for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+        np.testing.assert_array_equal(out[0][k], out[2][k], err_msg=k)
+        np.testing.assert_array_equal(out[1][k], out[3][k], err_msg="after jump " + k)
 
 
 def test_line_of_ten_permutation(gpu):
