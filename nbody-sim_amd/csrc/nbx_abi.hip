@@ -76,6 +76,10 @@ struct nbx_tree {
     // side stream of the collision pass, which overlaps the force walk
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // longest-first wave order of the stepping walk; lpt_tg: the walk shape
+    // its buffers hold
+    bool lpt = true;
+    int lpt_tg = 0;
     bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
     double *h_adj = nullptr;  // pinned copy of B.adj_p
 };
@@ -145,6 +149,9 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.spk, t->T.bkoff[t->T.nseg]);
     ALLOC(B.spi, t->T.bkoff[t->T.nseg]);
     ALLOC(B.bm, t->T.moff[t->T.nseg]);
+    B.lpt_stride = walk_waves(t->T, 8);
+    ALLOC(B.lpt_order, 2 * B.lpt_stride);
+    ALLOC(B.lpt_cost, 2 * B.lpt_stride);
     ALLOC(B.status[0], 2048 * B.n_sweep_tiles);
     ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
     ALLOC(B.tickets, 32);
@@ -201,7 +208,7 @@ void free_bufs(nbx_tree *t)
                     B.step_ctr, B.dtab,    B.prefix2,
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
-                    B.bm};
+                    B.bm,      B.lpt_order, B.lpt_cost};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
@@ -875,7 +882,9 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     const int has_drag = sp->gas_drag_rate > 0.0f;
     const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
     WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr};
+               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr,
+               t->lpt ? 1 : 0};
+    const bool overlap = !t->timing;
     if (e0 >= 0) ev_record(t, e0 + 0);
     if (t->compensated)
         launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
@@ -884,22 +893,29 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
         launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
                                p->az, dth, dt, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 1);
-    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false);
+    // the build also sorts the next walk's wave order (from the last walk's
+    // wave costs), in a spare block of its key pass
+    LptArgs lpt;
+    if (t->lpt) {
+        lpt.nw = walk_waves(t->T, t->walk_tg);
+        lpt.step_ctr = t->B.step_ctr;
+    }
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false, lpt);
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);
     // The collision pass reads the tree and the velocities and writes only
     // B.dv; the walk writes only the accelerations: without phase timing the
     // collision pass runs on the side stream, filling the walk's idle slots.
-    const bool overlap = sp->collisions && !t->timing;
-    if (overlap) (void)hipEventRecord(t->fork, t->stream);
+    const bool side_collide = sp->collisions && overlap;
+    if (side_collide) (void)hipEventRecord(t->fork, t->stream);
     launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
-    if (overlap) {  // submitted after the walk, so the walk's waves go first
+    if (side_collide) {  // submitted after the walk, so the walk's waves go first
         (void)hipStreamWaitEvent(t->side, t->fork, 0);
         launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->side);
         (void)hipEventRecord(t->join, t->side);
     }
     if (e0 >= 0) ev_record(t, e0 + 3);
-    if (overlap)
+    if (side_collide)
         (void)hipStreamWaitEvent(t->stream, t->join, 0);
     else if (sp->collisions)
         launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
@@ -946,6 +962,10 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     rc = upload_steps(t, dp, step0, n_steps, sp->dt, false);
     if (rc) return rc;
     const DiskTable D = disk_rcoll(t, dp);
+    if (t->lpt && t->lpt_tg != t->walk_tg) {  // wave orders of another shape: start over
+        reset_lpt(t->B, t->stream);
+        t->lpt_tg = t->walk_tg;
+    }
     const bool graph = sp->use_graph && !t->timing;
     if (graph) {
         std::vector<unsigned char> key = step_key(p, sp, D, t->B, t->walk_tg);

@@ -97,4 +97,82 @@ __device__ inline void perturber_add_dev(float xi, float yi, float zi, float px,
     az += f * dz;
 }
 
+// ---------------------------------------------------------------------------
+// Wave order of the next stepping walk: a counting sort of the wave costs of
+// the last walk, longest first (any permutation is correct; the order only
+// shortens the tail of the launch).  Step parity p = *step_ctr & 1: the walk
+// of a step reads order[p] and writes cost[p]; this sort, run during the
+// tree build of the same step, reads cost[p ^ 1] (the previous walk) and
+// writes order[p ^ 1] (the next walk).  One block of NT threads.
+// ---------------------------------------------------------------------------
+constexpr int kLptBins = 1024;
+constexpr int kLptCache = 4096;  // costs held in registers by the block (the rest reloads)
+
+// Monotone bin of a cost: exact below 64, then 32 bins per octave (< 864 in
+// all).  Bins are listed in descending order, so the largest costs go first.
+__device__ inline int lpt_bin(int cost)
+{
+    const int c = max(cost, 0);
+    int b = c;
+    if (c >= 64) {
+        const int e = 31 - __clz(c);
+        b = 64 + (e - 6) * 32 + ((c >> (e - 5)) & 31);
+    }
+    return kLptBins - 1 - b;
+}
+
+template <int NT>
+__device__ inline void lpt_sort_block(const int *__restrict__ step_ctr,
+                                      const int *__restrict__ cost, int *__restrict__ order,
+                                      int stride, int nw)
+{
+    constexpr int NWV = NT / 64, BPT = kLptBins / NT, R = kLptCache / NT;
+    static_assert(kLptBins % NT == 0 && kLptCache % NT == 0, "items per thread");
+    __shared__ int s_h[kLptBins];
+    __shared__ int s_w[NWV];
+    const int t = threadIdx.x, lane = __lane_id(), wv = t / 64;
+    const int q = ((*step_ctr & 1) ^ 1) * stride;
+    const int *c = cost + q;
+    int *o = order + q;
+    int bn[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {  // one round of independent loads
+        const int w = t + r * NT;
+        bn[r] = w < nw ? lpt_bin(c[w]) : -1;
+    }
+    for (int b = t; b < kLptBins; b += NT) s_h[b] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (bn[r] >= 0) atomicAdd(&s_h[bn[r]], 1);
+    for (int w = t + kLptCache; w < nw; w += NT) atomicAdd(&s_h[lpt_bin(c[w])], 1);
+    __syncthreads();
+    // exclusive scan of the bins: BPT consecutive bins per thread
+    int v[BPT], sum = 0;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        v[u] = s_h[t * BPT + u];
+        sum += v[u];
+    }
+    int inc = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_w[wv] = inc;
+    __syncthreads();
+    int run = inc - sum;
+    for (int k = 0; k < wv; ++k) run += s_w[k];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+        s_h[t * BPT + u] = run;
+        run += v[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (bn[r] >= 0) o[atomicAdd(&s_h[bn[r]], 1)] = t + r * NT;
+    for (int w = t + kLptCache; w < nw; w += NT) o[atomicAdd(&s_h[lpt_bin(c[w])], 1)] = w;
+}
+
 }  // namespace nbx

@@ -25,6 +25,10 @@ struct Bufs {
     int *spi = nullptr;          // [buckets] body index of that splitter
     int *bm = nullptr;           // [moff] packed {offset << 16 | count} per (bucket, scatter block)
     bool split_ready = false;    // spk / spi hold the quantiles of the last splitter sort
+    // launch order of the stepping walk's waves, by step parity (2 x lpt_stride)
+    int *lpt_order = nullptr;    // [2][lpt_stride] wave of each launch slot
+    int *lpt_cost = nullptr;     // [2][lpt_stride] work of each wave in its last walk
+    int lpt_stride = 0;          // waves of the narrowest walk shape (8 targets)
     unsigned *status[2] = {};    // [n_sweep_tiles * 2048] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
     DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
@@ -129,10 +133,17 @@ void launch_diagnostics(const float *x, const float *y, const float *z, const fl
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
                  hipStream_t st);
 void launch_root(const SegTable &T, Bufs &B, hipStream_t st);
+// Wave-order sort of the next stepping walk carried by a build (nw waves of
+// the walk shape; nw = 0: none).
+struct LptArgs {
+    int nw = 0;
+    const int *step_ctr = nullptr;
+};
 // resample = false lets the splitter sort reuse the splitters of the previous
 // build (the step loop); every other build samples them afresh.
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st, bool resample = true);
+                  const float *mass, Bufs &B, hipStream_t st, bool resample = true,
+                  const LptArgs &lpt = LptArgs{});
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st);
 void launch_fill_int(int *p, int v, int n, hipStream_t st);
 void launch_fill_float(float *p, float v, int n, hipStream_t st);
@@ -144,6 +155,7 @@ struct WalkArgs {
     int targets;                  // targets per wavefront: 64, 32, 16 or 8
     float *ax, *ay, *az;          // original order, written with '='
     float *sax, *say, *saz;       // optional self-gravity outputs
+    int lpt = 0;                  // 1: launch waves longest-first (needs step_ctr)
 };
 // Perturber values per disk come from dtab[*step_ctr][disk] when dtab is
 // non-null (graph-replayable), otherwise no perturber.
@@ -155,6 +167,10 @@ void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, co
 void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
                         int *wstat, Bufs &B, hipStream_t st);
 int walk_waves(const SegTable &T, int targets);
+// Longest-first wave order for the next stepping walk (see k_lpt_order), and
+// its reset to the identity (when the walk shape changes).
+void launch_lpt_order(int nw, const int *step_ctr, Bufs &B, hipStream_t st);
+void reset_lpt(Bufs &B, hipStream_t st);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,
                     const float *vz, float restitution, Bufs &B, hipStream_t st);

@@ -447,8 +447,9 @@ __device__ __forceinline__ uint64_t morton_key(float px, float py, float pz, con
 
 // Root cubes of disks [s0, s1] into s_root (block 0: of every disk, also
 // published to roots[]), and the resets owed by the key pass.
+// nblocks: the blocks of the key pass (all call this).
 __device__ void keys_prologue(const SegTable &T, const float *__restrict__ partial,
-                              Root *__restrict__ roots, Root *s_root, int s0, int s1,
+                              Root *__restrict__ roots, Root *s_root, int s0, int s1, int nblocks,
                               unsigned *__restrict__ status0, int n_status,
                               unsigned long long *__restrict__ scanw, int n_scanw,
                               int *__restrict__ tickets, int *__restrict__ alloc)
@@ -459,7 +460,7 @@ __device__ void keys_prologue(const SegTable &T, const float *__restrict__ parti
     }
     for (int sg = s0; sg <= s1; ++sg) root_of_partials(T, partial, sg, &s_root[sg]);
     if (blockIdx.x == 0 && threadIdx.x < T.nseg) roots[threadIdx.x] = s_root[threadIdx.x];
-    const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = gridDim.x * kBlock;
+    const int gid = blockIdx.x * kBlock + threadIdx.x, gsz = nblocks * kBlock;
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
     for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
     if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
@@ -489,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
         const int b0 = blockIdx.x * kKeysPerBlock;
         const int b1 = min(n, b0 + kKeysPerBlock) - 1;
         keys_prologue(T, partial, roots, s_root, seg_of_body(T, b0), seg_of_body(T, b1),
-                      status0, n_status, scanw, n_scanw, tickets, alloc);
+                      gridDim.x, status0, n_status, scanw, n_scanw, tickets, alloc);
     }
     __syncthreads();
     float px[kKeysItems], py[kKeysItems], pz[kKeysItems];
@@ -997,15 +998,22 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SegTable T, const uint64_t *
     scatter_tile(T, s, kb, base, end, kk, spk, spi, skey, sidx, bm);
 }
 
-// Key pass and scatter in one kernel (splitters of the previous build).
+// Key pass and scatter in one kernel (splitters of the previous build).  With
+// lpt_nw > 0 one extra block sorts the wave order of the next walk.
 __global__ __launch_bounds__(kBlock) void k_keys_scatter(
     SegTable T, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ z, const float *__restrict__ partial, Root *__restrict__ roots,
     unsigned long long *__restrict__ scanw, int n_scanw, int *__restrict__ tickets,
     int *__restrict__ alloc, const uint64_t *__restrict__ spk, const int *__restrict__ spi,
-    uint64_t *__restrict__ skey, int *__restrict__ sidx, int *__restrict__ bm)
+    uint64_t *__restrict__ skey, int *__restrict__ sidx, int *__restrict__ bm,
+    const int *__restrict__ step_ctr, const int *__restrict__ lpt_cost,
+    int *__restrict__ lpt_order, int lpt_stride, int lpt_nw)
 {
     __shared__ Root s_root[kMaxSeg];
+    if (blockIdx.x == T.sblk[T.nseg]) {  // the extra block
+        lpt_sort_block<kBlock>(step_ctr, lpt_cost, lpt_order, lpt_stride, lpt_nw);
+        return;
+    }
     const int s = seg_of_sblock(T, blockIdx.x);
     const int kb = blockIdx.x - T.sblk[s];
     const int base = T.off[s] + kb * kSplitTile;
@@ -1018,7 +1026,8 @@ __global__ __launch_bounds__(kBlock) void k_keys_scatter(
         py[r] = i < end ? y[i] : 0.f;
         pz[r] = i < end ? z[i] : 0.f;
     }
-    keys_prologue(T, partial, roots, s_root, s, s, nullptr, 0, scanw, n_scanw, tickets, alloc);
+    keys_prologue(T, partial, roots, s_root, s, s, T.sblk[T.nseg], nullptr, 0, scanw, n_scanw,
+                  tickets, alloc);
     __syncthreads();
     uint64_t kk[kSplitItems];
 #pragma unroll
@@ -1678,10 +1687,13 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
 }
 
 static void launch_split_sort(const SegTable &T, const float *x, const float *y, const float *z,
-                              const float *mass, Bufs &B, bool resample, hipStream_t st)
+                              const float *mass, Bufs &B, bool resample, const LptArgs &lpt,
+                              hipStream_t st)
 {
     const int n = T.n_total;
     if (resample || !B.split_ready) {
+        if (lpt.nw > 0)
+            launch_lpt_order(lpt.nw, lpt.step_ctr, B, st);
         hipLaunchKernelGGL(k_keys<8>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
                            z, B.partial, B.roots, B.keys[0], nullptr, nullptr, nullptr, 0,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
@@ -1691,10 +1703,11 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
         hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0],
                            B.spk, B.spi, B.keys[1], B.vals[1], B.bm);
     } else {
-        hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                           B.partial, B.roots, (unsigned long long *)B.scanw,
-                           kScanWords * B.n_scan_tiles, B.tickets, B.alloc, B.spk, B.spi,
-                           B.keys[1], B.vals[1], B.bm);
+        hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg] + (lpt.nw > 0 ? 1 : 0)),
+                           dim3(kBlock), 0, st, T, x, y, z, B.partial, B.roots,
+                           (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
+                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.bm, lpt.step_ctr,
+                           B.lpt_cost, B.lpt_order, B.lpt_stride, lpt.nw);
     }
     hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads), 0, st, T,
                        B.keys[1], B.vals[1], B.bm, B.keys[0], B.vals[0], x, y, z, mass, B.bodies,
@@ -1704,11 +1717,12 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
 }
 
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st, bool resample)
+                  const float *mass, Bufs &B, hipStream_t st, bool resample, const LptArgs &lpt)
 {
     const int n = T.n_total;
+    if (B.radix_bits != 0 && lpt.nw > 0) launch_lpt_order(lpt.nw, lpt.step_ctr, B, st);
     switch (B.radix_bits) {
-    case 0: launch_split_sort(T, x, y, z, mass, B, resample, st); break;
+    case 0: launch_split_sort(T, x, y, z, mass, B, resample, lpt, st); break;
     case 9: launch_sort<9>(T, x, y, z, mass, B, st); break;
     case 11: launch_sort<11>(T, x, y, z, mass, B, st); break;
     default: launch_sort<8>(T, x, y, z, mass, B, st); break;

@@ -23,13 +23,16 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
-    float *saz, int *err)
+    float *saz, int *err, const int *__restrict__ order, int *__restrict__ cost, int lpt_stride)
 {
     using Lds = typename std::conditional<QUAD, WaveLdsQ, WaveLds>::type;
     __shared__ Lds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
-    const int w = blockIdx.x * kWalkWaves + wib;
-    if (w >= M.wave[T.nseg]) return;  // uniform per wave
+    const int slot = blockIdx.x * kWalkWaves + wib;
+    if (slot >= M.wave[T.nseg]) return;  // uniform per wave
+    // launch slot -> target group: longest waves of the previous step first
+    const int par = order ? (*step_ctr & 1) * lpt_stride : 0;
+    const int w = order ? order[par + slot] : slot;
     const int s = seg_of_wmap(T, M, w);
     const int lane = __lane_id(), tt = lane % TG, rr = lane / TG;
     const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
@@ -37,7 +40,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     const float4 b = bodies[valid ? pos : T.off[s]];
     GravityVisitor<QUAD> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
     const uint64_t active = ballot64(valid && rr == 0);  // bit tt per valid target
-    wave_walk<TG, QUAD, U>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
+    const int work = wave_walk<TG, QUAD, U>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
+    if (cost && lane == 0) cost[par + w] = work;
     const float fx = slot_sum<TG>(v.fx), fy = slot_sum<TG>(v.fy), fz = slot_sum<TG>(v.fz);
     if (!valid || rr != 0) return;
     const int i = perm[pos];
@@ -305,6 +309,13 @@ __global__ void k_iota(int *p, int n)
     if (i < n) p[i] = i;
 }
 
+// Identity order in both parity halves of the wave-order buffer.
+__global__ void k_iota2(int *p, int stride)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * stride) p[i] = i < stride ? i : i - stride;
+}
+
 __global__ void k_fof_final(int *parent, int *labels, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -330,10 +341,44 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     // with one target per lane a lane evaluates kUnroll64 records per step
     constexpr int U = TG >= 16 ? kUnroll64 : 1;
     auto kern = W.quad ? k_walk<TG, true, U> : k_walk<TG, false, U>;
+    const bool lpt = W.lpt && step_ctr;
     hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, M, dtab,
                        step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
                        B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay,
-                       W.az, W.sax, W.say, W.saz, B.err);
+                       W.az, W.sax, W.say, W.saz, B.err, lpt ? B.lpt_order : nullptr,
+                       lpt ? B.lpt_cost : nullptr, B.lpt_stride);
+}
+
+// ---------------------------------------------------------------------------
+// Wave order of the next walk: a counting sort of the wave costs of the last
+// walk, longest first.  Step parity p: the walk of a step reads order[p] and
+// writes cost[p]; this pass, run beside the tree build of the same step,
+// reads cost[p ^ 1] (the previous walk) and writes order[p ^ 1] (the next
+// walk).  Any permutation is correct; the order only shortens the tail of
+// the launch.
+// ---------------------------------------------------------------------------
+constexpr int kLptThreads = 1024;
+
+__global__ __launch_bounds__(kLptThreads) void k_lpt_order(const int *__restrict__ step_ctr,
+                                                           const int *__restrict__ cost,
+                                                           int *__restrict__ order, int stride,
+                                                           int nw)
+{
+    lpt_sort_block<kLptThreads>(step_ctr, cost, order, stride, nw);
+}
+
+void reset_lpt(Bufs &B, hipStream_t st)
+{
+    const int n = 2 * B.lpt_stride;
+    hipLaunchKernelGGL(k_iota2, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.lpt_order,
+                       B.lpt_stride);
+    launch_fill_int(B.lpt_cost, 0, n, st);
+}
+
+void launch_lpt_order(int nw, const int *step_ctr, Bufs &B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(kLptThreads), 0, st, step_ctr, B.lpt_cost,
+                       B.lpt_order, B.lpt_stride, nw);
 }
 
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,

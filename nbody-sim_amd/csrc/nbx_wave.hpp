@@ -87,10 +87,12 @@ struct Rec {
 // owns one target and evaluates every record of the batch.
 // Visitor: visit(rec, leaf) on the lanes that own the record's bit; returns
 // true when the lane opens the (internal) record.
+// Returns the wave's work in record steps plus a per-batch overhead (the
+// cost that orders the waves of the next launch, longest first).
 template <int TG, bool QUAD, int U = 1, class Lds, class Visitor>
-__device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
-                                 const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
-                                 int *err, Visitor &vis, int *wstat = nullptr)
+__device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__restrict__ rb,
+                                const float4 *__restrict__ rq, int root, uint64_t active, Lds &L,
+                                int *err, Visitor &vis, int *wstat = nullptr)
 {
     constexpr int LS = kWave / TG;
     constexpr uint64_t TMASK = TG == 64 ? ~0ull : ((1ull << TG) - 1ull);
@@ -100,7 +102,7 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
     // later reads, no cross-lane ordering is needed
     L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
     int sp = 1, sp_max = 1;
-    int n_pop = 0, n_batch = 0;  // records popped, batches (wave-uniform)
+    int n_pop = 0, n_batch = 0, n_step = 0;  // records popped, batches, steps (wave-uniform)
     // Take up to 64 records from the top blocks (from the end of a block).
     // The top entries are read in one parallel LDS access (lane j holds
     // entry sp-1-j) and walked with lane reads.
@@ -192,6 +194,7 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
 #pragma unroll
         for (int u = 0; u < U; ++u) stage(min(u * LS + rr, nrec - 1), cur[u], mcur[u]);
         for (int k0 = 0; k0 < nrec; k0 += SU) {
+            ++n_step;
             Rec nxt[U];
             uint64_t mnxt[U];
 #pragma unroll
@@ -243,6 +246,7 @@ __device__ inline void wave_walk(const float4 *__restrict__ ra, const int4 *__re
         wstat[1] = n_batch;
         wstat[2] = sp_max;
     }
+    return n_step + 4 * n_batch;
 }
 
 // Sum of a per-lane value over the LS record slots of target tt.
