@@ -15,7 +15,9 @@ Prints ONE JSON line.
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 _REPO = os.path.dirname(os.path.abspath(__file__))
@@ -57,6 +59,8 @@ def parse():
                     help="CPU baseline sample length in steps (0: about 15 s)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep leg")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--plain-steps", type=int, default=40,
+                    help="steps per disk of the reference-call-sequence sweep runs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the step's kernels directly instead of replaying a hipGraph")
     ap.add_argument("--no-window", action="store_true",
@@ -213,10 +217,36 @@ SWEEP_N = [16384, 32768, 65536, 98304, 131072, 262144]
 SWEEP_MJUP = [0.0, 0.3, 1.0, 3.0, 5.0, 10.0]
 
 
+def plain_driver_runs(nbx, steps):
+    """The six sweep disks run one after the other by the C++ driver with the
+    reference's own call sequence (main.cu:85-138, one synchronous library
+    call per kernel): wall time of each step loop, from the driver."""
+    runs = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for i, (n, mj) in enumerate(zip(SWEEP_N, SWEEP_MJUP)):
+            cfg = {"n_particles": n, "disk_r_min": 0.5, "disk_r_max": 4.0, "disk_h_factor": 0.01,
+                   "particle_mass_msun": DISK_MASS / n, "star_mass_msun": 1.0,
+                   "softening_AU": 1e-3, "theta": 0.5, "collision_radius_factor": 1.0,
+                   "restitution": 0.0, "gas_drag_rate": 1e-3, "dt": 0.01,
+                   "total_time": 1000.0, "output_every_n_steps": 1000000, "output_dir": tmp,
+                   "perturber_enabled": mj > 0, "perturber_mass_mjup": mj,
+                   "perturber_radius_AU": 5.2, "perturber_phase0": 0.0, "disk_seed": 100 + i}
+            path = os.path.join(tmp, f"disk{i}.json")
+            with open(path, "w") as f:
+                json.dump(cfg, f)
+            r = subprocess.run([nbx.RUN_PATH, path, "--plain", "--no-output", "--steps",
+                                str(steps)], capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError("plain driver run failed:\n" + r.stdout + r.stderr)
+            runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+    return runs
+
+
 def batched_sweep(nbx, np, torch, make_disk, args):
     """Six disks of uneven N, one perturber mass each, advanced together by
-    one replayed step (every phase one launch for all disks) against the
-    same six disks stepped one after the other as plain runs."""
+    one replayed step (every phase one launch for all disks), against the six
+    disks run one after the other (a) by the C++ driver with the reference's
+    own call sequence and (b) as captured single-disk steps of this library."""
     k = args.batch_steps
     disks = [make_disk(n, DISK_MASS, 0.5, 4.0, seed=100 + i) for i, n in enumerate(SWEEP_N)]
     sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
@@ -260,13 +290,25 @@ def batched_sweep(nbx, np, torch, make_disk, args):
     torch.cuda.synchronize()
     bat_s = time.perf_counter() - t0
     total = float(sum(SWEEP_N))
+    batched = total * k / bat_s
+    runs = plain_driver_runs(nbx, args.plain_steps)
+    plain_rate = sum(r["n"] * r["steps"] for r in runs) / sum(r["loop_s"] for r in runs)
     return {"disks": SWEEP_N, "perturber_mjup": SWEEP_MJUP, "steps": k,
             "walk_shape_batched": nbx.walk_shape(tb),
-            "walk_shape_plain": [nbx.walk_shape(t) for _, t, _ in plain],
-            "batched_particle_steps_per_s": total * k / bat_s,
-            "sequential_particle_steps_per_s": total * k / seq_s,
-            "speedup_vs_sequential": seq_s / bat_s,
-            "batched_ms_per_step": bat_s / k * 1e3, "sequential_ms_per_step": seq_s / k * 1e3}
+            "walk_shape_captured_runs": [nbx.walk_shape(t) for _, t, _ in plain],
+            "batched_particle_steps_per_s": batched,
+            "batched_ms_per_step": bat_s / k * 1e3,
+            "plain_runs": {"how": "C++ driver, reference call sequence main.cu:85-138, "
+                                  "one synchronous call per kernel",
+                           "steps_per_disk": args.plain_steps,
+                           "particle_steps_per_s": plain_rate,
+                           "ms_per_step_per_disk": [r["loop_s"] / r["steps"] * 1e3
+                                                    for r in runs]},
+            "speedup_vs_plain_runs": batched / plain_rate,
+            "captured_runs": {"how": "each disk alone, one replayed step graph",
+                              "particle_steps_per_s": total * k / seq_s,
+                              "ms_per_step": seq_s / k * 1e3},
+            "speedup_vs_captured_runs": seq_s / bat_s}
 
 
 def diff_window(nbx, np, torch, make_disk, args):

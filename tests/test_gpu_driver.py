@@ -78,3 +78,45 @@ def test_driver_run_matches_python_run(gpu, tmp_path):
         mine = [r for r in rows if r.split(",")[0] == str(k)]
         assert len(mine) == len(aggs) + 1  # aggregates, then the perturber row
         assert mine[-1].split(",")[4:6] == ["-1", "0"]
+
+
+def test_driver_plain_call_sequence(gpu, tmp_path):
+    """nbx_run --plain steps with the reference's own call sequence
+    (main.cu:85-138, one synchronous call per kernel): after 20 steps its
+    frame lies within 4e-6 of the oracle's restated reference loop (the
+    bound of the captured step, see test_step_vs_reference_loop), and
+    --no-output prints the step loop's wall time."""
+    import pyoracle as O
+    nbx = gpu
+    n = 4096
+    cfg = {"n_particles": n, "disk_r_min": 0.5, "disk_r_max": 4.0, "disk_h_factor": 0.01,
+           "particle_mass_msun": 1e-5 / n, "star_mass_msun": 1.0, "softening_AU": 1e-3,
+           "theta": 0.5, "collision_radius_factor": 1.0, "restitution": 0.0,
+           "gas_drag_rate": 1e-3, "dt": 0.01, "total_time": 0.2, "output_every_n_steps": 20,
+           "output_dir": str(tmp_path / "data"), "perturber_enabled": True,
+           "perturber_mass_mjup": 10.0, "perturber_radius_AU": 5.2, "perturber_phase0": 0.0}
+    path = tmp_path / "plain.json"
+    path.write_text(json.dumps(cfg))
+    print("driver --plain n=4096 20 steps", flush=True)
+    r = subprocess.run([nbx.RUN_PATH, str(path), "--plain", "--out", str(tmp_path / "run")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, _, _, frames = read_frames(tmp_path / "run" / "frames.bin")
+    assert len(frames) == 1
+    c = nbx.loadConfig(str(path))
+    d = nbx.initDisk(c)
+    r_coll = float(c.collision_radius_factor * nbx.config_particle_radius(c))
+    params = O.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, r_coll=r_coll,
+                           restitution=0.0, gas_drag_rate=1e-3, perturber_enabled=1,
+                           pert_mass_mjup=10.0, pert_radius_au=5.2, collisions=1,
+                           exact_collide=1)
+    s = O.ref_steps(d, d["mass"], params, 0, 20, init_forces=True)
+    xyz = frames[0][1]
+    for q, k in enumerate(("x", "y", "z")):
+        err = np.abs(xyz[q] - s[k]).max()
+        assert err <= 4e-6, (k, err)
+    r = subprocess.run([nbx.RUN_PATH, str(path), "--plain", "--no-output", "--steps", "5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    timing = json.loads(r.stdout.strip().splitlines()[-1])
+    assert timing["n"] == n and timing["steps"] == 5 and timing["loop_s"] > 0
