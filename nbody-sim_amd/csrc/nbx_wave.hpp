@@ -158,7 +158,8 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     fetch();
     while (nrec_n > 0) {
         const int nrec = nrec_n;
-        if (rec_n >= 0) {  // stage the batch that arrived
+        {  // stage the batch that arrived; a lane without a record stages a
+           // finite stale one under a zero mask, so no slot needs a bound test
             L.a[lane] = ga;
             L.b[lane] = gb;
             L.m[lane] = mask_n;
@@ -174,8 +175,7 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         fetch();
         // Record steps, software-pipelined: the staging entries of the next
         // step are read while this one is evaluated (the staging area does
-        // not change during the batch).  Slots past the batch read a valid
-        // entry and are masked off.
+        // not change during the batch).
         auto stage = [&](int k, Rec &r, uint64_t &m) {
             r.a = L.a[k];
             r.b = L.b[k];
@@ -187,25 +187,15 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
             }
         };
         // a step covers SU = LS * U records: slot rr of unroll u takes record
-        // k0 + u LS + rr, so every lane evaluates U independent records
+        // k0 + u LS + rr, so every lane evaluates U independent records.
+        // Slots past the batch carry zero masks; the last prefetch reads the
+        // final slot again and is discarded.
         constexpr int SU = LS * U;
-        Rec cur[U];
-        uint64_t mcur[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) stage(min(u * LS + rr, nrec - 1), cur[u], mcur[u]);
-        for (int k0 = 0; k0 < nrec; k0 += SU) {
-            ++n_step;
-            Rec nxt[U];
-            uint64_t mnxt[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                stage(min(k0 + SU + u * LS + rr, nrec - 1), nxt[u], mnxt[u]);
+        static_assert(SU <= kWave, "a step stays inside the staging area");
+        auto evaluate = [&](Rec(&cur)[U], const uint64_t(&mc)[U]) {
             bool open[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool mine = (k0 + u * LS + rr < nrec) && ((mcur[u] >> tt) & 1ull);
-                open[u] = vis.visit_masked(cur[u], mine);
-            }
+            for (int u = 0; u < U; ++u) open[u] = vis.visit_masked(cur[u], (mc[u] >> tt) & 1ull);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint64_t om = ballot64(open[u]);
@@ -229,11 +219,25 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
                     sp_max = max(sp_max, sp);
                 }
             }
+        };
+        // two register sets in turn (no copies between steps)
+        Rec ca[U], cb[U];
+        uint64_t ma[U], mb[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                cur[u] = nxt[u];
-                mcur[u] = mnxt[u];
-            }
+        for (int u = 0; u < U; ++u) stage(u * LS + rr, ca[u], ma[u]);
+        for (int k0 = 0;;) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) stage(min(k0 + SU + u * LS + rr, kWave - 1), cb[u], mb[u]);
+            evaluate(ca, ma);
+            ++n_step;
+            k0 += SU;
+            if (k0 >= nrec) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) stage(min(k0 + SU + u * LS + rr, kWave - 1), ca[u], ma[u]);
+            evaluate(cb, mb);
+            ++n_step;
+            k0 += SU;
+            if (k0 >= nrec) break;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -284,7 +288,8 @@ struct GravityVisitor {
         const bool leaf = b.w == 0;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        const bool open = mine && !leaf && !(__int_as_float(b.y) < theta2 * r2);
+        // a leaf stores h^2 = -1, so the test never opens it
+        const bool open = mine && !(__int_as_float(b.y) < theta2 * r2);
         const bool take = mine && !open && !(leaf && b.z == self);
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float rinv2 = rinv * rinv;
