@@ -1502,7 +1502,8 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     const int *__restrict__ octf, const int *__restrict__ cnt, const int *__restrict__ first,
     const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
     const float4 *__restrict__ bodies, const float4 *__restrict__ bin_q, float4 *rec_a,
-    int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r)
+    int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r,
+    int *opar, int *opar_leaf)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     // sorted-position range [lo, hi] covered by a record (adjoint only)
@@ -1534,6 +1535,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     if (j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
         float4 a, c;
         int4 b;
+        opar_leaf[T.off[j]] = -1;  // the body is the disk's root
         rec_of(~T.off[j], a, b, c);
         rec_a[T.noff[j]] = a;
         rec_b[T.noff[j]] = b;
@@ -1547,6 +1549,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     if (j == T.ioff[s]) {  // local node 0 is the disk's root
         float4 a, c;
         int4 b;
+        opar[j] = -1;
         rec_of(j, a, b, c);
         rec_a[T.noff[s]] = a;
         rec_b[T.noff[s]] = b;
@@ -1558,6 +1561,9 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     oct_children(left, right, octf, j, [&](int k, int code) {
         float4 a, c;
         int4 b;
+        // octree parent (binary index) of every record, for bottom-up searches
+        if (code < 0) opar_leaf[~code] = j;
+        else opar[code] = j;
         rec_of(code, a, b, c);
         rec_a[f0 + k] = a;
         rec_b[f0 + k] = b;
@@ -1748,7 +1754,8 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
                            B.first, B.alloc, B.maxlev, B.err);
     hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.left, B.right,
                        B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a,
-                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r);
+                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar,
+                       B.opar_leaf);
 }
 
 }  // namespace nbx

@@ -96,6 +96,16 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     if (n_open) n_open[i] = no;
 }
 
+// Tree arrays of the bottom-up start of the per-lane searches.
+struct SearchTree {
+    const int *opar, *opar_leaf, *first, *cnt;
+    const float4 *bin_cell;
+};
+static inline SearchTree search_bufs(const Bufs &B)
+{
+    return SearchTree{B.opar, B.opar_leaf, B.first, B.cnt, B.bin_cell};
+}
+
 // ---------------------------------------------------------------------------
 // Per-lane descent for small search radii.  A cell record {centre, h} holds
 // every body classified into it within h plus the rounding of the float
@@ -120,24 +130,40 @@ __device__ inline float root_guard(const Root &R)
     return (fabsf(R.c[0]) + fabsf(R.c[1]) + fabsf(R.c[2]) + R.c[3]) * 2e-6f;
 }
 
+// Lowest octree node (binary index) whose cell holds every body within
+// `radius` of the target at sorted position pos: the climb from the body's
+// own cell stops at the first cell whose box, shrunk by twice the descent
+// rounding, contains the search sphere (such a body is classified into that
+// cell), or at the root.  -1: the body is its disk's only body.
+__device__ inline int search_start(const int *__restrict__ opar, const int *__restrict__ opar_leaf,
+                                   const float4 *__restrict__ bin_cell, int pos, float tx,
+                                   float ty, float tz, float radius, float guard)
+{
+    int j = opar_leaf[pos];
+    if (j < 0) return -1;
+    const float reach = radius * 1.0001f + 2.f * guard;
+    for (;;) {
+        const float4 c = bin_cell[j];
+        if (fabsf(tx - c.x) + reach < c.w && fabsf(ty - c.y) + reach < c.w &&
+            fabsf(tz - c.z) + reach < c.w)
+            return j;
+        const int up = opar[j];
+        if (up < 0) return j;
+        j = up;
+    }
+}
+
+// Depth-first search of the records below one children block {first,
+// count}: the order in which the per-body search from the root meets the
+// pairs inside this block's node.
 template <class Pair>
-__device__ inline void lane_walk(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
-                                 int root, float tx, float ty, float tz, float radius,
-                                 float guard, int2 *st, int *err, Pair &pair)
+__device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
+                                       int first0, int count0, float tx, float ty, float tz,
+                                       float radius, float guard, int2 *st, int *err, Pair &pair)
 {
     // st: this lane's stack, stride kWave between entries (bank-conflict free)
-    int sp = 0;
-    {
-        const int4 b = rb[root];
-        const float4 c = rc[root];
-        if (b.w == 0) {
-            pair(c, b.z);
-            return;
-        }
-        if (!near_cell(tx, ty, tz, c, radius, guard)) return;
-        st[0] = make_int2(b.z, b.w);
-        sp = 1;
-    }
+    st[0] = make_int2(first0, count0);
+    int sp = 1;
     while (sp > 0) {
         --sp;
         const int2 e = st[sp * kWave];
@@ -200,11 +226,23 @@ struct CollidePair {
     }
 };
 
+// Pairs of the target at sorted position pos within `radius`: the search
+// below the lowest cell that holds them all.
+template <class Pair>
+__device__ inline void lane_search(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
+                                   const SearchTree &S, int pos, float tx, float ty, float tz,
+                                   float radius, float guard, int2 *st, int *err, Pair &pair)
+{
+    const int j = search_start(S.opar, S.opar_leaf, S.bin_cell, pos, tx, ty, tz, radius, guard);
+    if (j < 0) return;  // a disk of one body has no pairs
+    lane_walk_block(rc, rb, S.first[j], S.cnt[j], tx, ty, tz, radius, guard, st, err, pair);
+}
+
 __global__ __launch_bounds__(kWalkBlock) void k_collide(
     SegTable T, DiskTable D, const float4 *__restrict__ rc, const int4 *__restrict__ rb,
     const float4 *__restrict__ bodies, const int *__restrict__ perm,
     const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
-    const Root *__restrict__ roots, float restitution, float *dv, int *err)
+    const Root *__restrict__ roots, float restitution, float *dv, int *err, SearchTree S)
 {
     __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
@@ -219,8 +257,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(
     CollidePair v{b.x, b.y, b.z, r * r, restitution, vx[i], vy[i], vz[i], pos, perm, vx, vy, vz,
                   0.f, 0.f, 0.f, 0};
     if (r > 0.f)
-        lane_walk(rc, rb, T.noff[s], b.x, b.y, b.z, r, root_guard(roots[s]),
-                  st + wib * kLStack * kWave + __lane_id(), err, v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]),
+                    st + wib * kLStack * kWave + __lane_id(), err, v);
     const int n = T.n_total;
     dv[i] = v.dvx;
     dv[n + i] = v.dvy;
@@ -287,7 +325,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
                                                      const float4 *__restrict__ bodies,
                                                      const int *__restrict__ perm,
                                                      const Root *__restrict__ roots, float ll,
-                                                     int *parent, int *err)
+                                                     int *parent, int *err, SearchTree S)
 {
     __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
@@ -299,8 +337,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
     const float4 b = bodies[pos];
     FofPair v{b.x, b.y, b.z, ll * ll, perm[pos], perm, parent};
     if (ll > 0.f)
-        lane_walk(rc, rb, T.noff[s], b.x, b.y, b.z, ll, root_guard(roots[s]),
-                  st + wib * kLStack * kWave + __lane_id(), err, v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]),
+                    st + wib * kLStack * kWave + __lane_id(), err, v);
 }
 
 __global__ void k_iota(int *p, int n)
@@ -429,7 +467,7 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
     const int waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
                        B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], vx, vy, vz, B.roots, restitution,
-                       B.dv, B.err);
+                       B.dv, B.err, search_bufs(B));
 }
 
 void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t st)
@@ -437,7 +475,8 @@ void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t s
     const int n = T.n_total, waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels, n);
     hipLaunchKernelGGL(k_fof_link, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
-                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], B.roots, ll, B.labels, B.err);
+                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], B.roots, ll, B.labels, B.err,
+                       search_bufs(B));
     hipLaunchKernelGGL(k_fof_final, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels,
                        labels, n);
 }
