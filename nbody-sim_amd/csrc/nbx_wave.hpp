@@ -294,7 +294,9 @@ struct GravityVisitor {
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float rinv2 = rinv * rinv;
         const float rinv3 = rinv2 * rinv;
-        const float f = take ? a.w * rinv3 : 0.f;
+        // mass selected before the product: the force stays branch-free
+        const float mw = take ? a.w : 0.f;
+        const float f = mw * rinv3;
         fx += f * dx;
         fy += f * dy;
         fz += f * dz;
