@@ -149,6 +149,7 @@ int alloc_bufs(nbx_tree *t)
     ALLOC(B.spk, t->T.bkoff[t->T.nseg]);
     ALLOC(B.spi, t->T.bkoff[t->T.nseg]);
     ALLOC(B.bm, t->T.moff[t->T.nseg]);
+    ALLOC(B.spay, n);
     B.lpt_stride = walk_waves(t->T, 8);
     ALLOC(B.lpt_order, 2 * B.lpt_stride);
     ALLOC(B.lpt_cost, 2 * B.lpt_stride);
@@ -210,7 +211,7 @@ void free_bufs(nbx_tree *t)
                     B.step_ctr, B.dtab,    B.prefix2,
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
-                    B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf};
+                    B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);

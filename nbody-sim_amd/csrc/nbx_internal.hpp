@@ -32,7 +32,7 @@ constexpr int kBucketMean = 384;                   // target bodies per bucket
 constexpr int kMaxBuckets = 1024;                  // buckets per disk
 constexpr int kOversample = 8;                     // samples per bucket
 constexpr int kMaxSample = kOversample * kMaxBuckets;
-constexpr int kBucketCap = 4096;                   // largest bucket sorted in LDS
+constexpr int kBucketCap = 2048;                   // largest bucket sorted in LDS
 constexpr int kSplitMaxN = 1 << 20;                // larger disks use the radix passes
 constexpr int kMaxSplitBlocks = kSplitMaxN / kSplitTile;
 

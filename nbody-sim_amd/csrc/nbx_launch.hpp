@@ -24,6 +24,7 @@ struct Bufs {
     uint64_t *spk = nullptr;     // [buckets] lower splitter key of every bucket (first unused)
     int *spi = nullptr;          // [buckets] body index of that splitter
     int *bm = nullptr;           // [moff] packed {offset << 16 | count} per (bucket, scatter block)
+    float4 *spay = nullptr;      // [n] {x, y, z, m} staged beside keys[1] / vals[1]
     bool split_ready = false;    // spk / spi hold the quantiles of the last splitter sort
     // launch order of the stepping walk's waves, by step parity (2 x lpt_stride)
     int *lpt_order = nullptr;    // [2][lpt_stride] wave of each launch slot

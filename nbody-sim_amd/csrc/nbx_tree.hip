@@ -909,10 +909,12 @@ __global__ __launch_bounds__(kSplitThreads) void k_split(SegTable T,
 // end).  All threads of the block call it.
 __device__ __forceinline__ void scatter_tile(const SegTable &T, int s, int kb, int base, int end,
                                              const uint64_t (&kk)[kSplitItems],
+                                             const float4 (&pay)[kSplitItems],
                                              const uint64_t *__restrict__ spk,
                                              const int *__restrict__ spi,
                                              uint64_t *__restrict__ skey,
-                                             int *__restrict__ sidx, int *__restrict__ bm)
+                                             int *__restrict__ sidx, float4 *__restrict__ spay,
+                                             int *__restrict__ bm)
 {
     __shared__ uint64_t s_k[kMaxBuckets];
     __shared__ int s_i[kMaxBuckets];
@@ -976,38 +978,41 @@ __device__ __forceinline__ void scatter_tile(const SegTable &T, int s, int kb, i
         const int pos = base + s_o[bk[r]] + lr[r];
         skey[pos] = kk[r];
         sidx[pos] = base + r * kBlock + t;
+        spay[pos] = pay[r];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scatter(SegTable T, const uint64_t *__restrict__ keys,
-                                                    const uint64_t *__restrict__ spk,
-                                                    const int *__restrict__ spi,
-                                                    uint64_t *__restrict__ skey,
-                                                    int *__restrict__ sidx, int *__restrict__ bm)
+__global__ __launch_bounds__(kBlock) void k_scatter(
+    SegTable T, const uint64_t *__restrict__ keys, const float *__restrict__ x,
+    const float *__restrict__ y, const float *__restrict__ z, const float *__restrict__ m,
+    const uint64_t *__restrict__ spk, const int *__restrict__ spi, uint64_t *__restrict__ skey,
+    int *__restrict__ sidx, float4 *__restrict__ spay, int *__restrict__ bm)
 {
     const int s = seg_of_sblock(T, blockIdx.x);
     const int kb = blockIdx.x - T.sblk[s];
     const int base = T.off[s] + kb * kSplitTile;
     const int end = min(T.off[s + 1], base + kSplitTile);
     uint64_t kk[kSplitItems];
+    float4 pay[kSplitItems];
 #pragma unroll
     for (int r = 0; r < kSplitItems; ++r) {
         const int i = base + r * kBlock + threadIdx.x;
         kk[r] = i < end ? keys[i] : 0ull;
+        pay[r] = i < end ? make_float4(x[i], y[i], z[i], m[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    scatter_tile(T, s, kb, base, end, kk, spk, spi, skey, sidx, bm);
+    scatter_tile(T, s, kb, base, end, kk, pay, spk, spi, skey, sidx, spay, bm);
 }
 
 // Key pass and scatter in one kernel (splitters of the previous build).  With
 // lpt_nw > 0 one extra block sorts the wave order of the next walk.
 __global__ __launch_bounds__(kBlock) void k_keys_scatter(
     SegTable T, const float *__restrict__ x, const float *__restrict__ y,
-    const float *__restrict__ z, const float *__restrict__ partial, Root *__restrict__ roots,
-    unsigned long long *__restrict__ scanw, int n_scanw, int *__restrict__ tickets,
-    int *__restrict__ alloc, const uint64_t *__restrict__ spk, const int *__restrict__ spi,
-    uint64_t *__restrict__ skey, int *__restrict__ sidx, int *__restrict__ bm,
-    const int *__restrict__ step_ctr, const int *__restrict__ lpt_cost,
-    int *__restrict__ lpt_order, int lpt_stride, int lpt_nw)
+    const float *__restrict__ z, const float *__restrict__ m, const float *__restrict__ partial,
+    Root *__restrict__ roots, unsigned long long *__restrict__ scanw, int n_scanw,
+    int *__restrict__ tickets, int *__restrict__ alloc, const uint64_t *__restrict__ spk,
+    const int *__restrict__ spi, uint64_t *__restrict__ skey, int *__restrict__ sidx,
+    float4 *__restrict__ spay, int *__restrict__ bm, const int *__restrict__ step_ctr,
+    const int *__restrict__ lpt_cost, int *__restrict__ lpt_order, int lpt_stride, int lpt_nw)
 {
     __shared__ Root s_root[kMaxSeg];
     if (blockIdx.x == T.sblk[T.nseg]) {  // the extra block
@@ -1018,75 +1023,69 @@ __global__ __launch_bounds__(kBlock) void k_keys_scatter(
     const int kb = blockIdx.x - T.sblk[s];
     const int base = T.off[s] + kb * kSplitTile;
     const int end = min(T.off[s + 1], base + kSplitTile);
-    float px[kSplitItems], py[kSplitItems], pz[kSplitItems];
+    float px[kSplitItems], py[kSplitItems], pz[kSplitItems], pm[kSplitItems];
 #pragma unroll
     for (int r = 0; r < kSplitItems; ++r) {
         const int i = base + r * kBlock + threadIdx.x;
         px[r] = i < end ? x[i] : 0.f;
         py[r] = i < end ? y[i] : 0.f;
         pz[r] = i < end ? z[i] : 0.f;
+        pm[r] = i < end ? m[i] : 0.f;
     }
     keys_prologue(T, partial, roots, s_root, s, s, T.sblk[T.nseg], nullptr, 0, scanw, n_scanw,
                   tickets, alloc);
     __syncthreads();
     uint64_t kk[kSplitItems];
+    float4 pay[kSplitItems];
 #pragma unroll
-    for (int r = 0; r < kSplitItems; ++r) kk[r] = morton_key(px[r], py[r], pz[r], s_root[s]);
-    scatter_tile(T, s, kb, base, end, kk, spk, spi, skey, sidx, bm);
+    for (int r = 0; r < kSplitItems; ++r) {
+        kk[r] = morton_key(px[r], py[r], pz[r], s_root[s]);
+        pay[r] = make_float4(px[r], py[r], pz[r], pm[r]);
+    }
+    scatter_tile(T, s, kb, base, end, kk, pay, spk, spi, skey, sidx, spay, bm);
 }
 
 constexpr int kSortThreads = 512;
 
-template <int NT, class KP, class IP>
-__device__ __forceinline__ void gather_bucket(KP dk, IP di, const uint64_t *__restrict__ skey,
-                                              const int *__restrict__ sidx, const int *s_pre,
-                                              const int *s_src, int nblk, int c)
+// Staging position of bucket member j: contribution k with
+// s_pre[k] <= j < s_pre[k + 1] (top: the largest power of two <= nblk).
+__device__ __forceinline__ int member_src(const int *s_pre, const int *s_src, int nblk, int top,
+                                          int j)
 {
-    int top = 1;
-    while (2 * top <= nblk) top <<= 1;
-    for (int j = threadIdx.x; j < c; j += NT) {
-        // contribution k with s_pre[k] <= j < s_pre[k + 1]
-        int k = 0;
-        for (int step = top; step > 0; step >>= 1)
-            if (k + step < nblk && s_pre[k + step] <= j) k += step;
-        const int q = s_src[k] + (j - s_pre[k]);
-        dk[j] = skey[q];
-        di[j] = sidx[q];
-    }
+    int k = 0;
+    for (int step = top; step > 0; step >>= 1)
+        if (k + step < nblk && s_pre[k + step] <= j) k += step;
+    return s_src[k] + (j - s_pre[k]);
 }
 
-// Gathered bodies of a sorted bucket, and the splitters of the next build:
-// the pairs at the quantile positions floor(b n / nb) of the disk.
-template <int NT, class KP, class IP>
-__device__ __forceinline__ void bucket_emit(const SegTable &T, int s, KP dk, IP di, int c,
-                                            int earlier, const float *__restrict__ x,
-                                            const float *__restrict__ y,
-                                            const float *__restrict__ z,
-                                            const float *__restrict__ m,
-                                            float4 *__restrict__ bodies,
-                                            uint64_t *__restrict__ spk, int *__restrict__ spi)
+// Splitter of the next build at disk position p, if p is a quantile
+// position floor(b n / nb) (p nb and b n stay below 2^31: n <= kSplitMaxN,
+// nb <= kMaxBuckets).
+__device__ __forceinline__ void emit_splitter(const SegTable &T, int s, int p, uint64_t key, int v,
+                                              uint64_t *__restrict__ spk, int *__restrict__ spi)
 {
     const int n = T.off[s + 1] - T.off[s];
     const int nb = T.bkoff[s + 1] - T.bkoff[s];
-    for (int j = threadIdx.x; j < c; j += NT) {
-        const int v = di[j];
-        bodies[j] = make_float4(x[v], y[v], z[v], m[v]);
-        // p nb and q n stay below 2^31 (n <= kSplitMaxN, nb <= kMaxBuckets)
-        const int p = earlier + j;  // position inside the disk
-        const int q = (p * nb + n - 1) / n;
-        if (q >= 1 && q < nb && q * n / nb == p) {
-            spk[T.bkoff[s] + q] = dk[j];
-            spi[T.bkoff[s] + q] = v;
-        }
+    const int q = (p * nb + n - 1) / n;
+    if (q >= 1 && q < nb && q * n / nb == p) {
+        spk[T.bkoff[s] + q] = key;
+        spi[T.bkoff[s] + q] = v;
     }
 }
 
+// The bucket sort's sort value in LDS: (body index - disk offset) << 11 |
+// member slot.  Ordering by it is ordering by body index (n_s <= 2^20), and
+// the slot finds the member's staged {x, y, z, m} after sorting (c <= 2048).
+constexpr int kSlotBits = 11;
+static_assert(kBucketCap <= (1 << kSlotBits) && kSplitMaxN <= (1 << (31 - kSlotBits)),
+              "index and slot share 31 bits");
+
 __global__ __launch_bounds__(kSortThreads) void k_bucket_sort(
     SegTable T, const uint64_t *__restrict__ skey, const int *__restrict__ sidx,
-    const int *__restrict__ bm, uint64_t *__restrict__ kout, int *__restrict__ vout,
-    const float *__restrict__ x, const float *__restrict__ y, const float *__restrict__ z,
-    const float *__restrict__ m, float4 *__restrict__ bodies, uint64_t *__restrict__ spk,
-    int *__restrict__ spi)
+    const float4 *__restrict__ spay, const int *__restrict__ bm, uint64_t *__restrict__ kout,
+    int *__restrict__ vout, const float *__restrict__ x, const float *__restrict__ y,
+    const float *__restrict__ z, const float *__restrict__ m, float4 *__restrict__ bodies,
+    uint64_t *__restrict__ spk, int *__restrict__ spi)
 {
     __shared__ uint64_t sk[kBucketCap];
     __shared__ int si[kBucketCap];
@@ -1121,26 +1120,43 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_sort(
         run += cw[u] & 0xffff;
     }
     __syncthreads();
-    const int out = T.off[s] + earlier;
-    uint64_t *ok = kout + out;
-    int *ov = vout + out;
+    int top = 1;
+    while (2 * top <= nblk) top <<= 1;
+    const int off = T.off[s], out = off + earlier;
     if (c <= kBucketCap) {
-        gather_bucket<kSortThreads>(sk, si, skey, sidx, s_pre, s_src, nblk, c);
+        for (int j = t; j < c; j += kSortThreads) {
+            const int q = member_src(s_pre, s_src, nblk, top, j);
+            sk[j] = skey[q];
+            si[j] = ((sidx[q] - off) << kSlotBits) | j;
+        }
         __syncthreads();
         block_sort<kSortThreads>(sk, si, c);
         for (int j = t; j < c; j += kSortThreads) {
-            ok[j] = sk[j];
-            ov[j] = si[j];
+            const int w = si[j];
+            const int v = off + (w >> kSlotBits);
+            const uint64_t key = sk[j];
+            kout[out + j] = key;
+            vout[out + j] = v;
+            bodies[out + j] = spay[member_src(s_pre, s_src, nblk, top, w & ((1 << kSlotBits) - 1))];
+            emit_splitter(T, s, earlier + j, key, v, spk, spi);
         }
-        bucket_emit<kSortThreads>(T, s, sk, si, c, earlier, x, y, z, m, bodies + out, spk, spi);
-    } else {  // in place at the bucket's final range
-        gather_bucket<kSortThreads>(ok, ov, skey, sidx, s_pre, s_src, nblk, c);
+    } else {  // in place at the bucket's final range, by body index
+        uint64_t *ok = kout + out;
+        int *ov = vout + out;
+        for (int j = t; j < c; j += kSortThreads) {
+            const int q = member_src(s_pre, s_src, nblk, top, j);
+            ok[j] = skey[q];
+            ov[j] = sidx[q];
+        }
         __syncthreads();
         block_sort<kSortThreads>(ok, ov, c);
-        bucket_emit<kSortThreads>(T, s, ok, ov, c, earlier, x, y, z, m, bodies + out, spk, spi);
+        for (int j = t; j < c; j += kSortThreads) {
+            const int v = ov[j];
+            bodies[out + j] = make_float4(x[v], y[v], z[v], m[v]);
+            emit_splitter(T, s, earlier + j, ok[j], v, spk, spi);
+        }
     }
 }
-
 
 // ===========================================================================
 // Exclusive prefix P[i] = sum_{s<i} e(s) of per-body moment vectors of the
@@ -1706,18 +1722,18 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
                            B.alloc);
         hipLaunchKernelGGL(k_split, dim3(T.nseg), dim3(kSplitThreads), 0, st, T, B.keys[0],
                            B.spk, B.spi);
-        hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0],
-                           B.spk, B.spi, B.keys[1], B.vals[1], B.bm);
+        hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0], x,
+                           y, z, mass, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm);
     } else {
         hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg] + (lpt.nw > 0 ? 1 : 0)),
-                           dim3(kBlock), 0, st, T, x, y, z, B.partial, B.roots,
+                           dim3(kBlock), 0, st, T, x, y, z, mass, B.partial, B.roots,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
-                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.bm, lpt.step_ctr,
-                           B.lpt_cost, B.lpt_order, B.lpt_stride, lpt.nw);
+                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm,
+                           lpt.step_ctr, B.lpt_cost, B.lpt_order, B.lpt_stride, lpt.nw);
     }
     hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads), 0, st, T,
-                       B.keys[1], B.vals[1], B.bm, B.keys[0], B.vals[0], x, y, z, mass, B.bodies,
-                       B.spk, B.spi);
+                       B.keys[1], B.vals[1], B.spay, B.bm, B.keys[0], B.vals[0], x, y, z, mass,
+                       B.bodies, B.spk, B.spi);
     B.cur = 0;
     B.split_ready = true;
 }

@@ -140,7 +140,7 @@ def test_split_sort_unrepresentative_sample(gpu):
     print("split sort, clumped sample n=65536", flush=True)
     keys = _order_vs_oracle(nbx, d)
     clump = (keys >= keys[sel].min()) & (keys <= keys[sel].max())
-    assert (~clump).sum() > 2 * 4096  # the oversized buckets really occur
+    assert (~clump).sum() > 2 * 2048  # the oversized buckets really occur
 
 
 def test_split_sort_equal_keys(gpu):
