@@ -34,14 +34,16 @@ TRAFFIC_FILE = os.path.join(_REPO, "profiles", "traffic_c2.txt")
 
 
 def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the same
-    counter reads for a plain copy of known size); None when not measured."""
+    """HBM bytes per dispatch of `kernel` (a name up to a template argument,
+    e.g. "k_walk<32,false") from the committed counter summary (rocprofv3
+    --pmc FETCH_SIZE and WRITE_SIZE, each scaled by what the same counter
+    reads for a plain copy of known size); None when not measured."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) >= 6 and w[0] == "kernel" and w[1] == kernel:
+                if (len(w) >= 6 and w[0] == "kernel" and w[1].startswith(kernel)
+                        and w[1][len(kernel):len(kernel) + 1] in (",", ">", "")):
                     return float(w[3]) + float(w[5])
     except OSError:
         pass
@@ -179,7 +181,7 @@ def main():
     # contraction to put on the matrix cores), priced against the FP32 peak
     roof_walk = {"bound": "valu", "achieved": walk_tflops, "peak": MI355X_FP32_TFLOPS,
                  "unit": "TFLOP/s", "frac": walk_tflops / MI355X_FP32_TFLOPS,
-                 "traffic": measured_traffic(f"k_walk<{nbx.walk_shape(t)},false>"),
+                 "traffic": measured_traffic(f"k_walk<{nbx.walk_shape(t)},false"),
                  "kernel": "k_walk", "ms": phases["walk"]}
     roof_build = {"bound": "hbm", "achieved": build_gbps, "peak": MI355X_HBM_GBPS,
                   "unit": "GB/s", "frac": build_gbps / MI355X_HBM_GBPS, "traffic": None,
