@@ -174,6 +174,7 @@ int alloc_bufs(nbx_tree *t)
     B.maxlev = B.alloc + kMaxSeg;  // adjacent: cleared by one fill
     ALLOC(B.opar, n);
     ALLOC(B.opar_leaf, n);
+    ALLOC(B.kids, 8 * (size_t)n);
     ALLOC(B.bin_com, n);
     ALLOC(B.bin_cell, n);
     B.n_rec = t->T.noff[t->T.nseg] + kRecPad;
@@ -211,7 +212,8 @@ void free_bufs(nbx_tree *t)
                     B.step_ctr, B.dtab,    B.prefix2,
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
-                    B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay};
+                    B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay,
+                    B.kids};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);

@@ -145,7 +145,15 @@ __device__ inline void lpt_sort_block(const int *__restrict__ step_ctr,
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (bn[r] >= 0) atomicAdd(&s_h[bn[r]], 1);
-    for (int w = t + kLptCache; w < nw; w += NT) atomicAdd(&s_h[lpt_bin(c[w])], 1);
+    // beyond the cache: rounds of 8 independent loads per thread
+    for (int w0 = t + kLptCache; w0 < nw; w0 += 8 * NT) {
+        int bb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) bb[u] = w0 + u * NT < nw ? lpt_bin(c[w0 + u * NT]) : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (bb[u] >= 0) atomicAdd(&s_h[bb[u]], 1);
+    }
     __syncthreads();
     // exclusive scan of the bins: BPT consecutive bins per thread
     int v[BPT], sum = 0;
@@ -172,7 +180,14 @@ __device__ inline void lpt_sort_block(const int *__restrict__ step_ctr,
 #pragma unroll
     for (int r = 0; r < R; ++r)
         if (bn[r] >= 0) o[atomicAdd(&s_h[bn[r]], 1)] = t + r * NT;
-    for (int w = t + kLptCache; w < nw; w += NT) o[atomicAdd(&s_h[lpt_bin(c[w])], 1)] = w;
+    for (int w0 = t + kLptCache; w0 < nw; w0 += 8 * NT) {
+        int bb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) bb[u] = w0 + u * NT < nw ? lpt_bin(c[w0 + u * NT]) : -1;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (bb[u] >= 0) o[atomicAdd(&s_h[bb[u]], 1)] = w0 + u * NT;
+    }
 }
 
 }  // namespace nbx

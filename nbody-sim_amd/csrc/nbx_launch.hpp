@@ -73,6 +73,7 @@ struct Bufs {
     int2 *rec_r = nullptr;       // [n_rec] sorted-position range of walk records
     int *opar = nullptr;         // [n] octree parent (binary index) of octree nodes, -1: root
     int *opar_leaf = nullptr;    // [n] octree parent of every sorted body's leaf record
+    int *kids = nullptr;         // [8 n] children codes of every octree node (binary index)
     double *adj_g = nullptr;     // [3 n_rec] per-record sums of J(d) abar over accepting targets
     double *adj_d = nullptr;     // [3 (n + 1)] range-add differences over sorted positions
     double *adj_blk = nullptr;   // [3 blocks] block totals of the difference scan

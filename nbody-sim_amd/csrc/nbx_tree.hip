@@ -1467,12 +1467,14 @@ __device__ inline int oct_children(const int *__restrict__ left, const int *__re
 }
 
 // Child counts and record blocks: one ticketed atomic per block and disk.
+// The children found (at most 8 codes per octree node) are kept in kids[8 j ..]
+// for the record pass.
 __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__restrict__ left,
                                                       const int *__restrict__ right,
                                                       const int *__restrict__ octf,
                                                       const int *__restrict__ plen, int *cnt,
                                                       int *first, int *alloc, int *maxlev,
-                                                      int *err)
+                                                      int *err, int *__restrict__ kids)
 {
     __shared__ int s_base[kMaxSeg];
     __shared__ int s_tot[kMaxSeg];
@@ -1488,8 +1490,10 @@ __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__r
     if (j < ni) {
         s = seg_of_internal(T, j);
         if (octf[j]) {
-            c = oct_children(left, right, octf, j, [](int, int) {});
-            if (c < 0) {
+            c = oct_children(left, right, octf, j, [&](int k, int code) {
+                if (k < 8) kids[8 * j + k] = code;
+            });
+            if (c < 0 || c > 8) {
                 atomicOr(err, kErrTree);
                 c = 0;
             }
@@ -1514,12 +1518,11 @@ __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__r
 // Walk records: the children of octree node j into its block, the root
 // record of each disk at noff[s], and one-body disks (root = the body).
 __global__ __launch_bounds__(kBlock) void k_oct_fill(
-    SegTable T, const int *__restrict__ left, const int *__restrict__ right,
-    const int *__restrict__ octf, const int *__restrict__ cnt, const int *__restrict__ first,
+    SegTable T, const int *__restrict__ cnt, const int *__restrict__ first,
     const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
     const float4 *__restrict__ bodies, const float4 *__restrict__ bin_q, float4 *rec_a,
     int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r,
-    int *opar, int *opar_leaf)
+    int *opar, int *opar_leaf, const int *__restrict__ kids)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     // sorted-position range [lo, hi] covered by a record (adjoint only)
@@ -1573,20 +1576,25 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         put_q(j, T.noff[s]);
         put_r(j, T.noff[s]);
     }
-    const int f0 = first[j];
-    oct_children(left, right, octf, j, [&](int k, int code) {
+    const int f0 = first[j], nk = cnt[j];
+    int code[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) code[k] = k < nk ? kids[8 * j + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k >= nk) break;
         float4 a, c;
         int4 b;
         // octree parent (binary index) of every record, for bottom-up searches
-        if (code < 0) opar_leaf[~code] = j;
-        else opar[code] = j;
-        rec_of(code, a, b, c);
+        if (code[k] < 0) opar_leaf[~code[k]] = j;
+        else opar[code[k]] = j;
+        rec_of(code[k], a, b, c);
         rec_a[f0 + k] = a;
         rec_b[f0 + k] = b;
         rec_c[f0 + k] = c;
-        put_q(code, f0 + k);
-        put_r(code, f0 + k);
-    });
+        put_q(code[k], f0 + k);
+        put_r(code[k], f0 + k);
+    }
 }
 
 // ===========================================================================
@@ -1767,11 +1775,10 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
-                           B.first, B.alloc, B.maxlev, B.err);
-    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.left, B.right,
-                       B.oct, B.cnt, B.first, B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a,
-                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar,
-                       B.opar_leaf);
+                           B.first, B.alloc, B.maxlev, B.err, B.kids);
+    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.cnt, B.first,
+                       B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a, B.rec_b, B.rec_c,
+                       B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar, B.opar_leaf, B.kids);
 }
 
 }  // namespace nbx
