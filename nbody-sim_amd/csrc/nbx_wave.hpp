@@ -39,15 +39,24 @@ struct StackEntry {
     unsigned mlo, mhi;
 };
 
+// Staging slot of one popped record: {com, mass} and {mask bits 0..31, h^2,
+// first child, child count} (the record's h is not used by the walks, its
+// place carries the low half of the target mask).  Slots past the 64 of a
+// batch absorb the discarded prefetch of a batch's last step.
+struct StageSlot {
+    float4 a;
+    int4 b;
+};
+constexpr int kStageSlots = kWave + 8;
+
 struct WaveLds {
     StackEntry st[kGStack];
-    float4 a[kWave];
-    int4 b[kWave];
-    unsigned long long m[kWave];
+    StageSlot s[kStageSlots];
+    unsigned mhi[kStageSlots];  // mask bits 32..63 (read with 64 targets per wave)
 };
 
 struct WaveLdsQ : WaveLds {
-    float4 q[2 * kWave];
+    float4 q[2 * kStageSlots];
 };
 
 // Squared distance of the opening test with every rounding fixed (no
@@ -62,7 +71,7 @@ __device__ inline float dist2(float dx, float dy, float dz)
 // Staging with the record index of every slot (visitors that write per
 // record, such as the adjoint's node accumulators).
 struct WaveLdsId : WaveLds {
-    int id[kWave];
+    int id[kStageSlots];
 };
 
 // A visitor that declares `static constexpr bool kRecId = true` receives the
@@ -160,9 +169,9 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         const int nrec = nrec_n;
         {  // stage the batch that arrived; a lane without a record stages a
            // finite stale one under a zero mask, so no slot needs a bound test
-            L.a[lane] = ga;
-            L.b[lane] = gb;
-            L.m[lane] = mask_n;
+            L.s[lane].a = ga;
+            L.s[lane].b = make_int4((int)(unsigned)mask_n, gb.y, gb.z, gb.w);
+            if constexpr (TG > 32) L.mhi[lane] = (unsigned)(mask_n >> 32);
             if constexpr (wants_rec_id<Visitor>::value) L.id[lane] = rec_n;
             if constexpr (QUAD) {
                 L.q[2 * lane] = gq0;
@@ -177,9 +186,10 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         // step are read while this one is evaluated (the staging area does
         // not change during the batch).
         auto stage = [&](int k, Rec &r, uint64_t &m) {
-            r.a = L.a[k];
-            r.b = L.b[k];
-            m = L.m[k];
+            r.a = L.s[k].a;
+            r.b = L.s[k].b;
+            m = (unsigned)r.b.x;
+            if constexpr (TG > 32) m |= (uint64_t)L.mhi[k] << 32;
             if constexpr (wants_rec_id<Visitor>::value) r.id = L.id[k];
             if constexpr (QUAD) {
                 r.q0 = L.q[2 * k];
@@ -188,10 +198,11 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         };
         // a step covers SU = LS * U records: slot rr of unroll u takes record
         // k0 + u LS + rr, so every lane evaluates U independent records.
-        // Slots past the batch carry zero masks; the last prefetch reads the
-        // final slot again and is discarded.
+        // Slots past the batch carry zero masks; the last prefetch reads past
+        // the batch and is discarded.
         constexpr int SU = LS * U;
-        static_assert(SU <= kWave, "a step stays inside the staging area");
+        static_assert(kWave % SU == 0 && kWave + SU <= kStageSlots,
+                      "the last prefetch stays inside the staging area");
         auto evaluate = [&](Rec(&cur)[U], const uint64_t(&mc)[U]) {
             bool open[U];
 #pragma unroll
@@ -227,13 +238,13 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         for (int u = 0; u < U; ++u) stage(u * LS + rr, ca[u], ma[u]);
         for (int k0 = 0;;) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) stage(min(k0 + SU + u * LS + rr, kWave - 1), cb[u], mb[u]);
+            for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, cb[u], mb[u]);
             evaluate(ca, ma);
             ++n_step;
             k0 += SU;
             if (k0 >= nrec) break;
 #pragma unroll
-            for (int u = 0; u < U; ++u) stage(min(k0 + SU + u * LS + rr, kWave - 1), ca[u], ma[u]);
+            for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, ca[u], ma[u]);
             evaluate(cb, mb);
             ++n_step;
             k0 += SU;
