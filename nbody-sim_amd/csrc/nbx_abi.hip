@@ -153,6 +153,7 @@ int alloc_bufs(nbx_tree *t)
     B.lpt_stride = walk_waves(t->T, 8);
     ALLOC(B.lpt_order, 2 * B.lpt_stride);
     ALLOC(B.lpt_cost, 2 * B.lpt_stride);
+    ALLOC(B.lpt_hist, (size_t)lpt_parts(B.lpt_stride) * kLptBins);
     ALLOC(B.status[0], 2048 * B.n_sweep_tiles);
     ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
     ALLOC(B.tickets, 32);
@@ -213,7 +214,7 @@ void free_bufs(nbx_tree *t)
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
                     B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay,
-                    B.kids};
+                    B.kids,    B.lpt_hist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
@@ -890,21 +891,18 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
                sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr,
                t->lpt ? 1 : 0};
     const bool overlap = !t->timing;
+    // the next walk's wave order, from the last walk's wave costs: counted by
+    // the kick-drift pass, placed by spare blocks of the key pass
+    const LptDev lpt = t->lpt ? lpt_dev(t->B, walk_waves(t->T, t->walk_tg), t->B.step_ctr)
+                              : LptDev{};
     if (e0 >= 0) ev_record(t, e0 + 0);
     if (t->compensated)
         launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
-                                 p->az, dth, dt, t->B, t->stream);
+                                 p->az, dth, dt, t->B, t->stream, lpt);
     else
         launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
-                               p->az, dth, dt, t->B, t->stream);
+                               p->az, dth, dt, t->B, t->stream, lpt);
     if (e0 >= 0) ev_record(t, e0 + 1);
-    // the build also sorts the next walk's wave order (from the last walk's
-    // wave costs), in a spare block of its key pass
-    LptArgs lpt;
-    if (t->lpt) {
-        lpt.nw = walk_waves(t->T, t->walk_tg);
-        lpt.step_ctr = t->B.step_ctr;
-    }
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false, lpt);
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);

@@ -101,12 +101,17 @@ __device__ inline void perturber_add_dev(float xi, float yi, float zi, float px,
 // Wave order of the next stepping walk: a counting sort of the wave costs of
 // the last walk, longest first (any permutation is correct; the order only
 // shortens the tail of the launch).  Step parity p = *step_ctr & 1: the walk
-// of a step reads order[p] and writes cost[p]; this sort, run during the
-// tree build of the same step, reads cost[p ^ 1] (the previous walk) and
-// writes order[p ^ 1] (the next walk).  One block of NT threads.
+// of a step reads order[p] and writes cost[p]; the sort, spread over the
+// step's kernels, reads cost[p ^ 1] (the previous walk) and writes
+// order[p ^ 1] (the next walk).  The waves are cut into parts of
+// kLptPerBlock:
+//   lpt_count    block `part` of the kick-drift pass writes the histogram of
+//                its part's costs to hist[part][.] (LDS counters);
+//   lpt_scatter  spare block `part` of the key pass derives its offsets from
+//                the table (bin totals over all parts, counts of the earlier
+//                parts) and places its waves with LDS counters.
+// No global atomics; every histogram row is rewritten each step.
 // ---------------------------------------------------------------------------
-constexpr int kLptBins = 1024;
-constexpr int kLptCache = 4096;  // costs held in registers by the block (the rest reloads)
 
 // Monotone bin of a cost: exact below 64, then 32 bins per octave (< 864 in
 // all).  Bins are listed in descending order, so the largest costs go first.
@@ -121,47 +126,62 @@ __device__ inline int lpt_bin(int cost)
     return kLptBins - 1 - b;
 }
 
-template <int NT>
-__device__ inline void lpt_sort_block(const int *__restrict__ step_ctr,
-                                      const int *__restrict__ cost, int *__restrict__ order,
-                                      int stride, int nw)
+// Called by every block (kBlock threads) of the kick-drift pass.
+__device__ inline void lpt_count(const LptDev &L)
 {
-    constexpr int NWV = NT / 64, BPT = kLptBins / NT, R = kLptCache / NT;
-    static_assert(kLptBins % NT == 0 && kLptCache % NT == 0, "items per thread");
+    const int part = blockIdx.x;
+    if (L.nw <= 0 || part >= lpt_parts(L.nw)) return;  // uniform per block
+    constexpr int R = kLptPerBlock / kBlock;
     __shared__ int s_h[kLptBins];
-    __shared__ int s_w[NWV];
-    const int t = threadIdx.x, lane = __lane_id(), wv = t / 64;
-    const int q = ((*step_ctr & 1) ^ 1) * stride;
-    const int *c = cost + q;
-    int *o = order + q;
-    int bn[R];
+    const int t = threadIdx.x;
+    const int q = ((*L.step_ctr & 1) ^ 1) * L.stride;
+    int bin[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {  // one round of independent loads
-        const int w = t + r * NT;
-        bn[r] = w < nw ? lpt_bin(c[w]) : -1;
+    for (int r = 0; r < R; ++r) {
+        const int w = part * kLptPerBlock + r * kBlock + t;
+        bin[r] = w < L.nw ? lpt_bin(L.cost[q + w]) : -1;
     }
-    for (int b = t; b < kLptBins; b += NT) s_h[b] = 0;
+    for (int b = t; b < kLptBins; b += kBlock) s_h[b] = 0;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (bn[r] >= 0) atomicAdd(&s_h[bn[r]], 1);
-    // beyond the cache: rounds of 8 independent loads per thread
-    for (int w0 = t + kLptCache; w0 < nw; w0 += 8 * NT) {
-        int bb[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) bb[u] = w0 + u * NT < nw ? lpt_bin(c[w0 + u * NT]) : -1;
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (bb[u] >= 0) atomicAdd(&s_h[bb[u]], 1);
-    }
+        if (bin[r] >= 0) atomicAdd(&s_h[bin[r]], 1);
     __syncthreads();
-    // exclusive scan of the bins: BPT consecutive bins per thread
-    int v[BPT], sum = 0;
+    for (int b = t; b < kLptBins; b += kBlock) L.hist[part * kLptBins + b] = s_h[b];
+}
+
+// Called by all NT threads of the block (after every lpt_count has run).
+template <int NT>
+__device__ inline void lpt_scatter(const LptDev &L, int part)
+{
+    constexpr int BPT = kLptBins / NT, R = kLptPerBlock / NT;
+    static_assert(kLptBins % NT == 0 && kLptPerBlock % NT == 0, "items per thread");
+    __shared__ int s_off[kLptBins];
+    __shared__ int s_w[NT / 64];
+    const int t = threadIdx.x, lane = __lane_id(), wv = t / 64;
+    const int nparts = lpt_parts(L.nw);
+    const int q = ((*L.step_ctr & 1) ^ 1) * L.stride;
+    int bin[R];
 #pragma unroll
-    for (int u = 0; u < BPT; ++u) {
-        v[u] = s_h[t * BPT + u];
-        sum += v[u];
+    for (int r = 0; r < R; ++r) {  // this part's waves, loads in flight during the table sums
+        const int w = part * kLptPerBlock + r * NT + t;
+        bin[r] = w < L.nw ? lpt_bin(L.cost[q + w]) : -1;
     }
+    int tot[BPT], bef[BPT];
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) tot[u] = bef[u] = 0;
+#pragma unroll 4
+    for (int p = 0; p < nparts; ++p) {
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) {
+            const int c = L.hist[p * kLptBins + t * BPT + u];
+            tot[u] += c;
+            bef[u] += p < part ? c : 0;
+        }
+    }
+    int sum = 0;
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) sum += tot[u];
     int inc = sum;
     for (int off = 1; off < 64; off <<= 1) {
         const int y = __shfl_up(inc, off, 64);
@@ -173,21 +193,13 @@ __device__ inline void lpt_sort_block(const int *__restrict__ step_ctr,
     for (int k = 0; k < wv; ++k) run += s_w[k];
 #pragma unroll
     for (int u = 0; u < BPT; ++u) {
-        s_h[t * BPT + u] = run;
-        run += v[u];
+        s_off[t * BPT + u] = run + bef[u];
+        run += tot[u];
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (bn[r] >= 0) o[atomicAdd(&s_h[bn[r]], 1)] = t + r * NT;
-    for (int w0 = t + kLptCache; w0 < nw; w0 += 8 * NT) {
-        int bb[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) bb[u] = w0 + u * NT < nw ? lpt_bin(c[w0 + u * NT]) : -1;
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (bb[u] >= 0) o[atomicAdd(&s_h[bb[u]], 1)] = w0 + u * NT;
-    }
+        if (bin[r] >= 0) L.order[q + atomicAdd(&s_off[bin[r]], 1)] = part * kLptPerBlock + r * NT + t;
 }
 
 }  // namespace nbx

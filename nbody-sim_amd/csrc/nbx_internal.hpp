@@ -94,6 +94,21 @@ __host__ __device__ inline int seg_of_internal(const SegTable &T, int j)
     return s;
 }
 
+// Wave-order sort of the next stepping walk (device view; the kernels that
+// run its phases are listed with lpt_count / lpt_scatter).
+constexpr int kLptBins = 1024;
+constexpr int kLptPerBlock = 1024;  // waves placed by one scatter block
+
+struct LptDev {
+    const int *step_ctr = nullptr;
+    const int *cost = nullptr;  // [2][stride]
+    int *order = nullptr;       // [2][stride]
+    int *hist = nullptr;        // [parts][kLptBins] cost histogram of every part
+    int stride = 0, nw = 0;     // nw = 0: no sort
+};
+
+__host__ __device__ inline int lpt_parts(int nw) { return (nw + kLptPerBlock - 1) / kLptPerBlock; }
+
 // Per-disk physical parameters of one step.
 struct DiskDev {
     float r_coll;

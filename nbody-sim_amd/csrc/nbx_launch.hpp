@@ -30,6 +30,7 @@ struct Bufs {
     int *lpt_order = nullptr;    // [2][lpt_stride] wave of each launch slot
     int *lpt_cost = nullptr;     // [2][lpt_stride] work of each wave in its last walk
     int lpt_stride = 0;          // waves of the narrowest walk shape (8 targets)
+    int *lpt_hist = nullptr;     // [parts][kLptBins] cost histogram of every part
     unsigned *status[2] = {};    // [n_sweep_tiles * 2048] look-back words per pass
     int *tickets = nullptr;      // [32] tile tickets (per pass / scan) + counters
     DVec<4> *prefix = nullptr;   // [n + 1] exclusive prefix of {m, m x, m y, m z}
@@ -110,7 +111,8 @@ void launch_perturber(float *ax, float *ay, float *az, const float *x, const flo
 // v += a*dt_half; x += v*dt; then the per-block boxes of the new x.
 void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, float *vx,
                             float *vy, float *vz, const float *ax, const float *ay,
-                            const float *az, float dt_half, float dt, Bufs &B, hipStream_t st);
+                            const float *az, float dt_half, float dt, Bufs &B, hipStream_t st,
+                            const LptDev &lpt = LptDev{});
 // v += dv (if dv); v += a*dt_half; gas drag (if decay < 1 or has_drag).
 // step_ctr (optional) is advanced by one after the kernel.
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -122,7 +124,8 @@ void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const fl
 // X - round(X); updates are evaluated in FP64 on X = hi + lo.
 void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,
                               float *vy, float *vz, const float *ax, const float *ay,
-                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st);
+                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st,
+                              const LptDev &lpt = LptDev{});
 void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const float *ay,
                         const float *az, const float *dv, const float *x, const float *z,
                         float dt_half, int has_drag, double decay, float star_mass, int n,
@@ -137,17 +140,12 @@ void launch_diagnostics(const float *x, const float *y, const float *z, const fl
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
                  hipStream_t st);
 void launch_root(const SegTable &T, Bufs &B, hipStream_t st);
-// Wave-order sort of the next stepping walk carried by a build (nw waves of
-// the walk shape; nw = 0: none).
-struct LptArgs {
-    int nw = 0;
-    const int *step_ctr = nullptr;
-};
 // resample = false lets the splitter sort reuse the splitters of the previous
-// build (the step loop); every other build samples them afresh.
+// build (the step loop); every other build samples them afresh.  lpt: the
+// placement phase of the next walk's wave order (nw = 0: none).
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
                   const float *mass, Bufs &B, hipStream_t st, bool resample = true,
-                  const LptArgs &lpt = LptArgs{});
+                  const LptDev &lpt = LptDev{});
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st);
 void launch_fill_int(int *p, int v, int n, hipStream_t st);
 void launch_fill_float(float *p, float v, int n, hipStream_t st);
@@ -171,9 +169,9 @@ void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, co
 void launch_walk_counts(const SegTable &T, float theta2, int targets, int *n_acc, int *n_open,
                         int *wstat, Bufs &B, hipStream_t st);
 int walk_waves(const SegTable &T, int targets);
-// Longest-first wave order for the next stepping walk (see k_lpt_order), and
-// its reset to the identity (when the walk shape changes).
-void launch_lpt_order(int nw, const int *step_ctr, Bufs &B, hipStream_t st);
+// Device view of the wave-order sort for a walk of nw waves; the reset puts
+// the identity in both orders (when the walk shape changes).
+LptDev lpt_dev(Bufs &B, int nw, const int *step_ctr);
 void reset_lpt(Bufs &B, hipStream_t st);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,

@@ -88,9 +88,10 @@ __device__ void bbox_post(float lx, float ly, float lz, float hx, float hy, floa
 __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox(
     SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
     const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
-    float dt_half, float dt, float *partial)
+    float dt_half, float dt, float *partial, LptDev L)
 {
 #pragma clang fp contract(off)
+    lpt_count(L);
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
     const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
@@ -149,8 +150,9 @@ __global__ void k_kick_drag(float *vx, float *vy, float *vz, const float *__rest
 __global__ __launch_bounds__(kBlock) void k_kick_drift_bbox_c(
     SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
     const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
-    float dt_half, float dt, float *lo, float *partial)
+    float dt_half, float dt, float *lo, float *partial, LptDev L)
 {
+    lpt_count(L);
     const int b = blockIdx.x;
     const int s = seg_of_block(T, b);
     const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
@@ -1003,20 +1005,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter(
     scatter_tile(T, s, kb, base, end, kk, pay, spk, spi, skey, sidx, spay, bm);
 }
 
-// Key pass and scatter in one kernel (splitters of the previous build).  With
-// lpt_nw > 0 one extra block sorts the wave order of the next walk.
+// Key pass and scatter in one kernel (splitters of the previous build); the
+// blocks past the scatter blocks place the next walk's wave order.
 __global__ __launch_bounds__(kBlock) void k_keys_scatter(
     SegTable T, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ z, const float *__restrict__ m, const float *__restrict__ partial,
     Root *__restrict__ roots, unsigned long long *__restrict__ scanw, int n_scanw,
     int *__restrict__ tickets, int *__restrict__ alloc, const uint64_t *__restrict__ spk,
     const int *__restrict__ spi, uint64_t *__restrict__ skey, int *__restrict__ sidx,
-    float4 *__restrict__ spay, int *__restrict__ bm, const int *__restrict__ step_ctr,
-    const int *__restrict__ lpt_cost, int *__restrict__ lpt_order, int lpt_stride, int lpt_nw)
+    float4 *__restrict__ spay, int *__restrict__ bm, LptDev L)
 {
     __shared__ Root s_root[kMaxSeg];
-    if (blockIdx.x == T.sblk[T.nseg]) {  // the extra block
-        lpt_sort_block<kBlock>(step_ctr, lpt_cost, lpt_order, lpt_stride, lpt_nw);
+    if (blockIdx.x >= T.sblk[T.nseg]) {  // uniform per block
+        lpt_scatter<kBlock>(L, blockIdx.x - T.sblk[T.nseg]);
         return;
     }
     const int s = seg_of_sblock(T, blockIdx.x);
@@ -1640,10 +1641,11 @@ void launch_fill_float(float *p, float v, int n, hipStream_t st)
 
 void launch_kick_drift_bbox(const SegTable &T, float *x, float *y, float *z, float *vx,
                             float *vy, float *vz, const float *ax, const float *ay,
-                            const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
+                            const float *az, float dt_half, float dt, Bufs &B, hipStream_t st,
+                            const LptDev &lpt)
 {
     hipLaunchKernelGGL(k_kick_drift_bbox, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial);
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.partial, lpt);
 }
 
 void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -1657,10 +1659,11 @@ void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const fl
 
 void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,
                               float *vy, float *vz, const float *ax, const float *ay,
-                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st)
+                              const float *az, float dt_half, float dt, Bufs &B, hipStream_t st,
+                              const LptDev &lpt)
 {
     hipLaunchKernelGGL(k_kick_drift_bbox_c, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
-                       vx, vy, vz, ax, ay, az, dt_half, dt, B.lo, B.partial);
+                       vx, vy, vz, ax, ay, az, dt_half, dt, B.lo, B.partial, lpt);
 }
 
 void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const float *ay,
@@ -1716,14 +1719,25 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
     B.cur = cur;
 }
 
+// The wave-order placement on its own (builds without the fused key pass).
+__global__ __launch_bounds__(kBlock) void k_lpt_scatter(LptDev L)
+{
+    lpt_scatter<kBlock>(L, blockIdx.x);
+}
+
+static void launch_lpt_scatter(const LptDev &lpt, hipStream_t st)
+{
+    if (lpt.nw > 0)
+        hipLaunchKernelGGL(k_lpt_scatter, dim3(lpt_parts(lpt.nw)), dim3(kBlock), 0, st, lpt);
+}
+
 static void launch_split_sort(const SegTable &T, const float *x, const float *y, const float *z,
-                              const float *mass, Bufs &B, bool resample, const LptArgs &lpt,
+                              const float *mass, Bufs &B, bool resample, const LptDev &lpt,
                               hipStream_t st)
 {
     const int n = T.n_total;
     if (resample || !B.split_ready) {
-        if (lpt.nw > 0)
-            launch_lpt_order(lpt.nw, lpt.step_ctr, B, st);
+        launch_lpt_scatter(lpt, st);
         hipLaunchKernelGGL(k_keys<8>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
                            z, B.partial, B.roots, B.keys[0], nullptr, nullptr, nullptr, 0,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
@@ -1733,11 +1747,10 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
         hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0], x,
                            y, z, mass, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm);
     } else {
-        hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg] + (lpt.nw > 0 ? 1 : 0)),
+        hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg] + lpt_parts(lpt.nw)),
                            dim3(kBlock), 0, st, T, x, y, z, mass, B.partial, B.roots,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
-                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm,
-                           lpt.step_ctr, B.lpt_cost, B.lpt_order, B.lpt_stride, lpt.nw);
+                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm, lpt);
     }
     hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads), 0, st, T,
                        B.keys[1], B.vals[1], B.spay, B.bm, B.keys[0], B.vals[0], x, y, z, mass,
@@ -1747,10 +1760,10 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
 }
 
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st, bool resample, const LptArgs &lpt)
+                  const float *mass, Bufs &B, hipStream_t st, bool resample, const LptDev &lpt)
 {
     const int n = T.n_total;
-    if (B.radix_bits != 0 && lpt.nw > 0) launch_lpt_order(lpt.nw, lpt.step_ctr, B, st);
+    if (B.radix_bits != 0) launch_lpt_scatter(lpt, st);
     switch (B.radix_bits) {
     case 0: launch_split_sort(T, x, y, z, mass, B, resample, lpt, st); break;
     case 9: launch_sort<9>(T, x, y, z, mass, B, st); break;

@@ -395,16 +395,6 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
 // walk).  Any permutation is correct; the order only shortens the tail of
 // the launch.
 // ---------------------------------------------------------------------------
-constexpr int kLptThreads = 1024;
-
-__global__ __launch_bounds__(kLptThreads) void k_lpt_order(const int *__restrict__ step_ctr,
-                                                           const int *__restrict__ cost,
-                                                           int *__restrict__ order, int stride,
-                                                           int nw)
-{
-    lpt_sort_block<kLptThreads>(step_ctr, cost, order, stride, nw);
-}
-
 void reset_lpt(Bufs &B, hipStream_t st)
 {
     const int n = 2 * B.lpt_stride;
@@ -413,10 +403,16 @@ void reset_lpt(Bufs &B, hipStream_t st)
     launch_fill_int(B.lpt_cost, 0, n, st);
 }
 
-void launch_lpt_order(int nw, const int *step_ctr, Bufs &B, hipStream_t st)
+LptDev lpt_dev(Bufs &B, int nw, const int *step_ctr)
 {
-    hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(kLptThreads), 0, st, step_ctr, B.lpt_cost,
-                       B.lpt_order, B.lpt_stride, nw);
+    LptDev L;
+    L.step_ctr = step_ctr;
+    L.cost = B.lpt_cost;
+    L.order = B.lpt_order;
+    L.hist = B.lpt_hist;
+    L.stride = B.lpt_stride;
+    L.nw = nw;
+    return L;
 }
 
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
