@@ -102,6 +102,45 @@ def cpu_baseline(d, r_coll, pert_mass_mjup, n_steps_hint):
                       f"collisions, drag, perturber) at N={n}, {el:.1f} s"}
 
 
+def fof_timing(nbx, np, torch, make_disk):
+    """FoF (Aggregate.cpp:26-76, link length r_coll) at N = 4,096 and 131,072:
+    the reference's own O(N^2) union-find (oracle/_ref, built from
+    Aggregate.cpp; single host thread) against the GPU partition of the same
+    positions (tree build + linking + labels, one synchronous call)."""
+    sys.path.insert(0, os.path.join(_REPO, "oracle"))
+    import pyoracle as O
+    ref = O.ref_lib()
+    out = {"cores": 1, "kind": "reference" if ref is not None else "port"}
+    for n, mass in ((4096, 1e-5), (131072, DISK_MASS)):
+        d = make_disk(n, mass, 0.5, 4.0, seed=2024)
+        ll = nbx.particlePhysicalRadius(np.float32(mass / n))
+        labels = np.empty(n, np.int32)
+        t0 = time.perf_counter()
+        if ref is not None:
+            agg_n = np.zeros(n, np.int32)
+            agg_f = np.zeros(8 * n, np.float32)
+            ref.ref_detect_aggregates(n, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], ll,
+                                      np.float32(mass / n), labels, n, agg_n, agg_f)
+        else:
+            labels, _ = O.fof(d["x"], d["y"], d["z"], ll, method="n2")
+        cpu_s = time.perf_counter() - t0
+        p = nbx.to_gpu(d)
+        t = nbx.allocateOctree(n)
+        lab = torch.empty(n, dtype=torch.int32, device="cuda")
+        nbx.fof_into(t, p, ll, lab)  # warm-up
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            nbx.fof_into(t, p, ll, lab)
+        torch.cuda.synchronize()
+        gpu_s = (time.perf_counter() - t0) / reps
+        same = bool(np.array_equal(lab.cpu().numpy(), labels))
+        out[str(n)] = {"cpu_ms": cpu_s * 1e3, "gpu_ms": gpu_s * 1e3,
+                       "pairs_tested_cpu": n * (n - 1) // 2, "labels_equal": same}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -186,6 +225,24 @@ def main():
     roof_build = {"bound": "hbm", "achieved": build_gbps, "peak": MI355X_HBM_GBPS,
                   "unit": "GB/s", "frac": build_gbps / MI355X_HBM_GBPS, "traffic": None,
                   "kernel": "build phase", "ms": phases["build"]}
+    # achievable rates on this GPU: FMA chains and the walk's own record mix
+    # (same 20-flop pricing), and a device-to-device copy
+    fma_tf = nbx.probe_fp32(0)
+    mix_tf = nbx.probe_fp32(1)
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    copy_gbps = 5 * 2 * src.numel() * 4 / (time.perf_counter() - t0) / 1e9
+    del src, dst
+    achievable = {"fp32_fma_chain_tflops": fma_tf, "walk_record_mix_tflops": mix_tf,
+                  "hbm_copy_gbps": copy_gbps,
+                  "walk_frac_of_record_mix": walk_tflops / mix_tf if mix_tf > 0 else None,
+                  "build_frac_of_copy": build_gbps / copy_gbps if copy_gbps > 0 else None}
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -198,6 +255,7 @@ def main():
                    "launch": "direct" if args.no_graph else "hipGraph replay per step"},
         "roofline": roof_walk,
         "roofline_build": roof_build,
+        "achievable": achievable,
         "phases_ms": phases,
         "walk_shape": {"targets_per_wave": nbx.walk_shape(t),
                        "tuned_ms": {str(k): v for k, v in tune_ms.items()}},
@@ -210,6 +268,7 @@ def main():
     if not args.no_precision:
         out["precision_study"] = precision_study(nbx, np, torch, make_disk, args)
     if not args.no_cpu:
+        out["cpu_fof"] = fof_timing(nbx, np, torch, make_disk)
         out["cpu_baseline"] = cpu_baseline(make_disk(n, DISK_MASS, 0.5, 4.0, seed=2024), r_coll,
                                            10.0, args.cpu_steps)
     print(json.dumps(out))

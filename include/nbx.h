@@ -299,6 +299,12 @@ int nbx_tree_stats(nbx_tree *t, int32_t *n_nodes, int32_t *max_level);
  * writes *ms, or NBX_E_ARG for an unknown phase. */
 int nbx_tree_set_timing(nbx_tree *t, int on);
 int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms);
+/* Achievable FP32 rate of this GPU (no reference counterpart; measurement
+ * only): kind 0 times eight independent FMA chains per thread (2 flop per
+ * FMA), kind 1 the walk's record evaluation on registers over an LDS table
+ * (its distance, opening test, selects, rsqrt and accumulation, priced like
+ * the walk at 20 flop per record).  Writes TFLOP/s. */
+int nbx_probe_fp32(int kind, double *tflops);
 
 #ifdef __cplusplus
 }

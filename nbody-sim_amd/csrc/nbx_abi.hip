@@ -1193,6 +1193,19 @@ int nbx_tree_set_timing(nbx_tree *t, int on)
     return NBX_OK;
 }
 
+int nbx_probe_fp32(int kind, double *tflops)
+{
+    if (!tflops || (kind != 0 && kind != 1)) return fail(NBX_E_ARG, "probe kind %d", kind);
+    float *scratch = nullptr;
+    HIPCHK(hipMalloc((void **)&scratch, 256 * 4 * 8 * sizeof(float)));
+    const double v = probe_fp32(kind, scratch, nullptr);
+    HIPCHK(hipDeviceSynchronize());
+    (void)hipFree(scratch);
+    if (v < 0.0) return fail(NBX_E_HIP, "probe timing failed");
+    *tflops = v;
+    return NBX_OK;
+}
+
 int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms)
 {
     if (!t || !phase || !ms) return fail(NBX_E_ARG, "null argument");

@@ -361,6 +361,87 @@ __global__ void k_fof_final(int *parent, int *labels, int n)
     labels[i] = uf_find(parent, i);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 probes: the achievable rates the walk is compared with.
+// ---------------------------------------------------------------------------
+constexpr int kProbeBlock = 256;
+
+__global__ __launch_bounds__(kProbeBlock) void k_probe_fma(float *out, int iters, float a, float b)
+{
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (float)(threadIdx.x + k) * 1e-3f;
+    for (int i = 0; i < iters; ++i)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = fmaf(v[k], a, b);
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sum += v[k];
+    if (sum == 1234.5f) out[blockIdx.x] = sum;  // keeps the chains live
+}
+
+// The walk's record evaluation (GravityVisitor::visit_masked) for one target
+// per lane over a table of 64 records read with LDS broadcasts.
+__global__ __launch_bounds__(kProbeBlock) void k_probe_mix(float *out, int iters, float theta2,
+                                                           float eps2)
+{
+    __shared__ float4 ta[64];
+    __shared__ float th2[64];
+    const int t = threadIdx.x;
+    if (t < 64) {
+        const float u = (float)t;
+        ta[t] = make_float4(0.7f * u - 20.f, 0.05f * u, 0.3f * u - 9.f, 1e-6f * (u + 1.f));
+        th2[t] = (t & 3) == 0 ? -1.f : 0.01f * u;
+    }
+    __syncthreads();
+    const float bx = 0.01f * t, by = 0.001f * t, bz = -0.01f * t;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int i = 0; i < iters; ++i) {
+        const int k = i & 63;
+        const float4 a = ta[k];
+        const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
+        const float r2 = dist2(dx, dy, dz);
+        const bool open = !(th2[k] < theta2 * r2);
+        const bool take = !open && k != (t & 63);
+        const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
+        const float mw = take ? a.w : 0.f;
+        const float f = mw * (rinv * rinv * rinv);
+        fx += f * dx;
+        fy += f * dy;
+        fz += f * dz;
+    }
+    if (fx + fy + fz == 1234.5f) out[blockIdx.x] = fx;
+}
+
+double probe_fp32(int kind, float *scratch, hipStream_t st)
+{
+    const int blocks = 256 * 4 * 8;  // 8 waves per SIMD across the chip
+    const int iters = kind == 0 ? 1 << 14 : 1 << 13;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -1.0;
+    auto launch = [&]() {
+        if (kind == 0)
+            hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(kProbeBlock), 0, st, scratch,
+                               iters, 0.999f, 1e-3f);
+        else
+            hipLaunchKernelGGL(k_probe_mix, dim3(blocks), dim3(kProbeBlock), 0, st, scratch,
+                               iters, 0.25f, 1e-6f);
+    };
+    launch();  // warm-up
+    (void)hipEventRecord(e0, st);
+    const int reps = 3;
+    for (int r = 0; r < reps; ++r) launch();
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double per_iter = kind == 0 ? 16.0 : 20.0;  // flop per thread and iteration
+    const double flop = per_iter * iters * (double)blocks * kProbeBlock * reps;
+    return ms > 0.f ? flop / (ms * 1e-3) / 1e12 : -1.0;
+}
+
 // ===========================================================================
 // Launchers
 // ===========================================================================

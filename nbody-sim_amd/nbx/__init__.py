@@ -105,7 +105,7 @@ EXPORTS = [
     "nbx_build", "nbx_summarize", "nbx_sort", "nbx_sort_idx", "nbx_sorted_keys", "nbx_force",
     "nbx_force_ex", "nbx_collide", "nbx_perturber_force", "nbx_half_kick", "nbx_drift",
     "nbx_reset_acceleration", "nbx_gas_drag", "nbx_fof", "nbx_init_forces", "nbx_step",
-    "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_walk_counts",
+    "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_probe_fp32", "nbx_walk_counts",
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
     "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
     "nbx_tree_set_radix_bits", "nbx_tree_set_adjoint", "nbx_adj_force", "nbx_adj_step",
@@ -146,6 +146,7 @@ def lib():
             "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
             "nbx_tree_stats": [vp, vp, vp], "nbx_tree_set_timing": [vp, i32],
             "nbx_phase_ms": [vp, C.c_char_p, C.POINTER(f32)],
+            "nbx_probe_fp32": [i32, C.POINTER(C.c_double)],
             "nbx_walk_counts": [vp, f32, vp, vp],
             "nbx_tree_set_multipole": [vp, i32],
             "nbx_tree_set_precision": [vp, i32],
@@ -623,6 +624,13 @@ def set_timing(t, on):
 def phase_ms(t, phase):
     v = C.c_float()
     _check(lib().nbx_phase_ms(t.h, phase.encode(), C.byref(v)), "phase_ms")
+    return v.value
+
+
+def probe_fp32(kind):
+    """Achievable FP32 TFLOP/s: kind 0 FMA chains, 1 the walk's record mix."""
+    v = C.c_double()
+    _check(lib().nbx_probe_fp32(int(kind), C.byref(v)), "probe_fp32")
     return v.value
 
 
