@@ -13,7 +13,47 @@ __device__ inline uint64_t lanemask_lt()
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-__device__ inline uint64_t ballot64(bool p) { return __ballot(p); }
+// Lanes 0..lane (2 << 63 wraps to 0, giving all 64 bits).
+__device__ inline uint64_t lanemask_le() { return (2ull << __lane_id()) - 1ull; }
+
+__device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Inclusive add-scan over the 64 lanes of a wave: DPP row shifts by 1, 2, 4
+// and 8 scan each row of 16 lanes, the broadcasts of lane 15 (to the row
+// above, within each half) and of lane 31 (to the upper half) join the rows.
+__device__ inline int wave_incl_add(int v)
+{
+    const int l = __lane_id(), rl = l & 15;
+    int t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    if (rl >= 1) v += t;
+    t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    if (rl >= 2) v += t;
+    t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    if (rl >= 4) v += t;
+    t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    if (rl >= 8) v += t;
+    t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);  // row_bcast:15
+    if ((l & 31) >= 16) v += t;
+    t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);  // row_bcast:31
+    if (l >= 32) v += t;
+    return v;
+}
+
+// Number of set bits of m below this lane (v_mbcnt).
+__device__ inline int mbcnt64(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+}  // namespace nbx
+
+// v_writelane_b32 (the intrinsic has no clang builtin here): lane `lane` of
+// the result takes the wave-uniform `val`, every other lane keeps `old`.
+extern "C" __device__ int nbx_writelane_i32(int val, int lane, int old) __asm(
+    "llvm.amdgcn.writelane.i32");
+
+namespace nbx {
 
 __device__ inline int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
 

@@ -17,7 +17,7 @@
 
 namespace nbx {
 
-template <int TG, bool QUAD, int U>
+template <int TG, bool QUAD, int U, bool SELF>
 __global__ __launch_bounds__(kWalkBlock) void k_walk(
     SegTable T, WaveMap M, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
     const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
-    GravityVisitor<QUAD> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
+    GravityVisitor<QUAD, SELF> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
     const uint64_t active = ballot64(valid && rr == 0);  // bit tt per valid target
     const int work = wave_walk<TG, QUAD, U>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
     if (cost && lane == 0) cost[par + w] = work;
@@ -459,7 +459,10 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     const int waves = M.wave[T.nseg];
     // with one target per lane a lane evaluates kUnroll64 records per step
     constexpr int U = TG >= 16 ? kUnroll64 : 1;
-    auto kern = W.quad ? k_walk<TG, true, U> : k_walk<TG, false, U>;
+    // the own-leaf test is needed only without softening
+    auto kern = W.eps2 >= kNoSelfTestEps2
+                    ? (W.quad ? k_walk<TG, true, U, false> : k_walk<TG, false, U, false>)
+                    : (W.quad ? k_walk<TG, true, U, true> : k_walk<TG, false, U, true>);
     const bool lpt = W.lpt && step_ctr;
     hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, M, dtab,
                        step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,

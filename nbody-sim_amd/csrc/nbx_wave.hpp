@@ -53,6 +53,7 @@ struct WaveLds {
     StackEntry st[kGStack];
     StageSlot s[kStageSlots];
     unsigned mhi[kStageSlots];  // mask bits 32..63 (read with 64 targets per wave)
+    int own[kWave];             // pop: flags of the lanes that start a stack entry
 };
 
 struct WaveLdsQ : WaveLds {
@@ -94,8 +95,9 @@ struct Rec {
 // evaluated L records at a time, lane (t, r) taking record k0 + r for target
 // t.  Stack masks carry one bit per target.  With TG = 64 (L = 1) every lane
 // owns one target and evaluates every record of the batch.
-// Visitor: visit(rec, leaf) on the lanes that own the record's bit; returns
-// true when the lane opens the (internal) record.
+// Visitor: visit_masked(rec, mine) on every lane; mine says whether the
+// lane's target owns the record.  The lane opens the (internal) record when
+// mine and the returned value are both true.
 // Returns the wave's work in record steps plus a per-batch overhead (the
 // cost that orders the waves of the next launch, longest first).
 template <int TG, bool QUAD, int U = 1, class Lds, class Visitor>
@@ -112,36 +114,39 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
     int sp = 1, sp_max = 1;
     int n_pop = 0, n_batch = 0, n_step = 0;  // records popped, batches, steps (wave-uniform)
-    // Take up to 64 records from the top blocks (from the end of a block).
-    // The top entries are read in one parallel LDS access (lane j holds
-    // entry sp-1-j) and walked with lane reads.
+    // Take up to 64 records from the top blocks (from the end of a block),
+    // all lanes at once: lane j holds entry sp-1-j and the prefix sum of the
+    // counts places entry j's records on lanes [excl_j, incl_j); the entry
+    // crossing lane 64 keeps its first records on the stack.  A lane finds
+    // the entry it serves from the flags of the entries' first lanes.
     auto pop = [&](int &nrec, int &my_rec, uint64_t &my_mask) {
-        nrec = 0;
-        my_rec = -1;
-        my_mask = 0;
         const bool deep = sp > kBatchMaxSp;
-        const int ne = min(sp, kWave);
+        const int ne = deep ? min(sp, 1) : min(sp, kWave);
         StackEntry top{0, 0, 0u, 0u};
         if (lane < ne) top = L.st[sp - 1 - lane];
-        int used = 0;
-        for (int j = 0; j < ne && nrec < kWave; ++j) {
-            const int count = __builtin_amdgcn_readlane(top.count, j);
-            const int first = __builtin_amdgcn_readlane(top.first, j);
-            const unsigned mlo = __builtin_amdgcn_readlane(top.mlo, j);
-            const unsigned mhi = __builtin_amdgcn_readlane(top.mhi, j);
-            const int take = min(count, kWave - nrec);
-            if (lane >= nrec && lane < nrec + take) {
-                my_rec = first + (count - take) + (lane - nrec);
-                my_mask = ((uint64_t)mhi << 32) | (uint64_t)mlo;
-            }
-            if (take == count) {
-                ++used;
-            } else {
-                L.st[sp - 1 - j].count = count - take;  // stays on top
-            }
-            nrec += take;
-            if (deep) break;
+        const int incl = wave_incl_add(top.count);  // entries keep count >= 1
+        const int excl = incl - top.count;
+        nrec = min(__builtin_amdgcn_readlane(incl, kWave - 1), kWave);
+        const int used = __popcll(ballot64(lane < ne && incl <= kWave));
+        L.own[lane] = 0;
+        if (lane < ne && excl < kWave) L.own[excl] = 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint64_t starts = ballot64(L.own[lane] != 0);
+        const int j = min(max(__popcll(starts & lanemask_le()) - 1, 0), ne - 1);
+        const int ej = __shfl(excl, j, kWave);
+        my_rec = -1;
+        my_mask = 0;
+        if (lane < nrec) {
+            const StackEntry e = L.st[sp - 1 - j];
+            const int take = min(e.count, kWave - ej);
+            const int off = min(max(e.count - take + (lane - ej), 0), e.count - 1);
+            my_rec = e.first + off;
+            my_mask = ((uint64_t)e.mhi << 32) | (uint64_t)e.mlo;
         }
+        // (the partial entry is rewritten after every lane has read it)
+        if (lane == used && lane < ne && excl < kWave) L.st[sp - 1 - lane].count = incl - kWave;
         sp -= used;
         n_pop += nrec;
         n_batch += nrec > 0;
@@ -153,22 +158,26 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     uint64_t mask_n;
     float4 ga = make_float4(0.f, 0.f, 0.f, 0.f), gq0 = ga, gq1 = ga;
     int4 gb = make_int4(0, 0, 0, 0);
+    // Every lane loads: a lane without a record reads the disk's root record
+    // (finite, staged under a zero mask), so the loads need no merge with the
+    // previous values and stay in flight until the batch is staged.
     auto fetch = [&]() {
         pop(nrec_n, rec_n, mask_n);
-        if (rec_n >= 0) {
-            ga = ra[rec_n];
-            gb = rb[rec_n];
-            if constexpr (QUAD) {
-                gq0 = rq[2 * rec_n];
-                gq1 = rq[2 * rec_n + 1];
-            }
+        const int ri = rec_n >= 0 ? rec_n : root;
+        ga = ra[ri];
+        gb = rb[ri];
+        if constexpr (QUAD) {
+            gq0 = rq[2 * ri];
+            gq1 = rq[2 * ri + 1];
         }
     };
     fetch();
     while (nrec_n > 0) {
         const int nrec = nrec_n;
-        {  // stage the batch that arrived; a lane without a record stages a
-           // finite stale one under a zero mask, so no slot needs a bound test
+        // child block of the record this lane staged (pushed after the batch)
+        const int rf = gb.z, rc = gb.w;
+        {  // stage the batch that arrived; a lane without a record stages the
+           // root record under a zero mask, so no slot needs a bound test
             L.s[lane].a = ga;
             L.s[lane].b = make_int4((int)(unsigned)mask_n, gb.y, gb.z, gb.w);
             if constexpr (TG > 32) L.mhi[lane] = (unsigned)(mask_n >> 32);
@@ -203,31 +212,26 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         constexpr int SU = LS * U;
         static_assert(kWave % SU == 0 && kWave + SU <= kStageSlots,
                       "the last prefetch stays inside the staging area");
-        auto evaluate = [&](Rec(&cur)[U], const uint64_t(&mc)[U]) {
-            bool open[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) open[u] = vis.visit_masked(cur[u], (mc[u] >> tt) & 1ull);
+        // Open masks of the batch: lane k of om_r (bits 32..63 in om_h)
+        // receives the mask of the targets that opened record k.  Steps only
+        // record them; the children are pushed once, after the batch.
+        int om_r = 0, om_h = 0;
+        auto evaluate = [&](Rec(&cur)[U], const uint64_t(&mc)[U], int k0) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const uint64_t om = ballot64(open[u]);
-                if (om) {
+                const bool mine = TG > 32 ? ((mc[u] >> tt) & 1ull) != 0
+                                          : (((unsigned)mc[u] >> tt) & 1u) != 0;
+                // (two ballots of plain compares: each is the compare's own
+                // lane mask, with no select in between)
+                const uint64_t om = ballot64(mine) & ballot64(vis.visit_masked(cur[u], mine));
 #pragma unroll
-                    for (int q = 0; q < LS; ++q) {
-                        const uint64_t mq = (om >> (q * TG)) & TMASK;
-                        if (mq) {
-                            // every lane of slot q holds record k0 + u LS + q
-                            const int first = __builtin_amdgcn_readlane(cur[u].b.z, q * TG);
-                            const int count = __builtin_amdgcn_readlane(cur[u].b.w, q * TG);
-                            if (sp < kGStack) {
-                                L.st[sp] = StackEntry{first, count, (unsigned)mq,
-                                                      (unsigned)(mq >> 32)};
-                                ++sp;
-                            } else if (lane == 0) {
-                                atomicOr(err, kErrStack);
-                            }
-                        }
-                    }
-                    sp_max = max(sp_max, sp);
+                for (int q = 0; q < LS; ++q) {
+                    // slot q of unroll u holds record k0 + u LS + q (< 64)
+                    const int k = k0 + u * LS + q;
+                    const uint64_t mq = (om >> (q * TG)) & TMASK;
+                    om_r = nbx_writelane_i32((int)(unsigned)mq, k, om_r);
+                    if constexpr (TG > 32)
+                        om_h = nbx_writelane_i32((int)(unsigned)(mq >> 32), k, om_h);
                 }
             }
         };
@@ -236,19 +240,38 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         uint64_t ma[U], mb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) stage(u * LS + rr, ca[u], ma[u]);
+        // (the scheduling barriers keep the next step's LDS reads ahead of
+        // this step's arithmetic)
         for (int k0 = 0;;) {
 #pragma unroll
             for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, cb[u], mb[u]);
-            evaluate(ca, ma);
+            __builtin_amdgcn_sched_barrier(0);
+            evaluate(ca, ma, k0);
             ++n_step;
             k0 += SU;
             if (k0 >= nrec) break;
 #pragma unroll
             for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, ca[u], ma[u]);
-            evaluate(cb, mb);
+            __builtin_amdgcn_sched_barrier(0);
+            evaluate(cb, mb, k0);
             ++n_step;
             k0 += SU;
             if (k0 >= nrec) break;
+        }
+        {  // push the child blocks of the opened records in record order
+           // (record k above record k - 1), lane k writing record k's entry
+            // (an entry always holds records: the pop relies on counts >= 1)
+            const bool opened = lane < nrec && rc > 0 && (om_r != 0 || om_h != 0);
+            const uint64_t fl = ballot64(opened);
+            if (fl) {
+                const int np = __popcll(fl);
+                const int at = sp + mbcnt64(fl);
+                if (opened && at < kGStack)
+                    L.st[at] = StackEntry{rf, rc, (unsigned)om_r, (unsigned)om_h};
+                if (sp + np > kGStack && lane == 0) atomicOr(err, kErrStack);
+                sp = min(sp + np, kGStack);
+                sp_max = max(sp_max, sp);
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -284,8 +307,14 @@ __device__ inline int slot_sum_i(int v)
 // Gravity: accept a record when it is a leaf or h^2 < theta^2 r^2 (the
 // half-width / COM-distance test of BHTree.cu:576-577); skip the target's
 // own leaf (BHTree.cu:580); monopole pull m d / (r^2+eps^2)^{3/2}.
+// SELF = false drops the own-leaf test: a leaf record holds its body's
+// position bit for bit, so for the own leaf d = +0 and the pull m d rinv^3
+// is +0 whenever m rinv^3 is finite, which the launcher ensures by taking
+// this form only for eps^2 >= kNoSelfTestEps2.
 // ---------------------------------------------------------------------------
-template <bool QUAD>
+constexpr float kNoSelfTestEps2 = 1e-12f;
+
+template <bool QUAD, bool SELF = true>
 struct GravityVisitor {
     float bx, by, bz, theta2, eps2;
     int self;
@@ -296,18 +325,18 @@ struct GravityVisitor {
     {
         const float4 &a = r.a;
         const int4 &b = r.b;
-        const bool leaf = b.w == 0;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        // a leaf stores h^2 = -1, so the test never opens it
-        const bool open = mine && !(__int_as_float(b.y) < theta2 * r2);
-        const bool take = mine && !open && !(leaf && b.z == self);
+        // a leaf stores h^2 = -1, so it is always accepted
+        const bool acc = __int_as_float(b.y) < theta2 * r2;
+        bool take = mine & acc;
+        if constexpr (SELF) take = take & !(b.w == 0 && b.z == self);
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float rinv2 = rinv * rinv;
         const float rinv3 = rinv2 * rinv;
-        // mass selected before the product: the force stays branch-free
-        const float mw = take ? a.w : 0.f;
-        const float f = mw * rinv3;
+        // the pull is selected, not the mass: without softening the own
+        // leaf has rinv = inf and m rinv^3 would be 0 inf = NaN
+        const float f = take ? a.w * rinv3 : 0.f;
         fx += f * dx;
         fy += f * dy;
         fz += f * dz;
@@ -318,12 +347,13 @@ struct GravityVisitor {
             const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
             const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
             const float rinv5 = take ? rinv3 * rinv2 : 0.f;
-            const float g = 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2;
+            const float g = take ? 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2 : 0.f;
             fx += g * dx - rinv5 * qx;
             fy += g * dy - rinv5 * qy;
             fz += g * dz - rinv5 * qz;
         }
-        return open;
+        // the walk takes the open mask as ballot(mine) & ballot(result)
+        return !acc;
     }
 };
 

@@ -257,9 +257,11 @@ def test_force_vs_oracle(gpu, n, mass, seed, shape):
     assert np.median(ed) <= 0.1, np.median(ed)
 
 
-def test_force_two_bodies_exact_pair(gpu):
+@pytest.mark.parametrize("soft", [1e-3, 0.0])
+def test_force_two_bodies_exact_pair(gpu, soft):
     """Two bodies: leaves are always accepted, so the tree term is the exact
-    softened pair force."""
+    (softened) pair force; without softening the own leaf must be skipped
+    (its 0/0 term would be NaN)."""
     nbx = gpu
     import torch
     f = np.float32
@@ -270,10 +272,30 @@ def test_force_two_bodies_exact_pair(gpu):
     p = nbx.to_gpu(d)
     t = _build(nbx, p)
     sa = [torch.zeros(2, dtype=torch.float32, device="cuda") for _ in range(3)]
-    nbx.launchForceKernel(t, p, 0.5, 1e-3, 1.0, self_acc=sa)
+    nbx.launchForceKernel(t, p, 0.5, soft, 1.0, self_acc=sa)
     g = np.stack([_np(s) for s in sa], 1)
-    ad = O.direct_f64(d["x"], d["y"], d["z"], d["mass"], 1e-3)
+    ad = O.direct_f64(d["x"], d["y"], d["z"], d["mass"], soft)
     assert rel_err(g, ad).max() <= 1e-6
+
+
+def test_force_unsoftened_vs_oracle(gpu):
+    """Softening 0 (the walk form with the own-leaf test) on a 4096-body disk:
+    tree self-gravity vs the oracle's walk of the same octree, median rel
+    diff <= 2e-6 and 99.9th pct <= 1e-3, every value finite."""
+    import torch
+    nbx = gpu
+    n = 4096
+    d = disk(n, mass=1e-5, seed=3)
+    p = nbx.to_gpu(d)
+    t = _build(nbx, p)
+    sa = [torch.zeros(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    nbx.launchForceKernel(t, p, 0.5, 0.0, 1.0, self_acc=sa)
+    g = np.stack([_np(s) for s in sa], 1)
+    assert np.isfinite(g).all()
+    mt = O.mt_force(d["x"], d["y"], d["z"], d["mass"], 0.5, 0.0)
+    e = rel_err(g, mt["self_a"])
+    assert np.median(e) <= 2e-6, np.median(e)
+    assert np.percentile(e, 99.9) <= 1e-3, np.percentile(e, 99.9)
 
 
 # ---------------------------------------------------------------------------
