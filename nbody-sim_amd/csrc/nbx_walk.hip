@@ -143,12 +143,13 @@ __device__ inline int search_start(const int *__restrict__ opar, const int *__re
     if (j < 0) return -1;
     const float reach = radius * 1.0001f + 2.f * guard;
     for (;;) {
+        // one round trip per level: the whole cell and the parent are read
+        // together, and the test does not short-circuit
         const float4 c = bin_cell[j];
-        if (fabsf(tx - c.x) + reach < c.w && fabsf(ty - c.y) + reach < c.w &&
-            fabsf(tz - c.z) + reach < c.w)
-            return j;
         const int up = opar[j];
-        if (up < 0) return j;
+        const bool inside = (fabsf(tx - c.x) + reach < c.w) & (fabsf(ty - c.y) + reach < c.w) &
+                            (fabsf(tz - c.z) + reach < c.w);
+        if (inside || up < 0) return j;
         j = up;
     }
 }
