@@ -1358,6 +1358,53 @@ __device__ inline int lcp(const uint64_t *__restrict__ k, int n, int a, int b)
     return __clzll(ka ^ kb);
 }
 
+// lcp(k[i], k[b]) for k[i] = ki from the key kb read at b's clamped index;
+// -1 for b outside [0, n).
+__device__ inline int lcp_kb(int n, int i, uint64_t ki, int b, uint64_t kb)
+{
+    const int v = ki == kb ? 64 + __clz((unsigned)(i ^ b)) : __clzll(ki ^ kb);
+    return (b < 0 || b >= n) ? -1 : v;
+}
+__device__ inline uint64_t key_at(const uint64_t *__restrict__ k, int n, int b)
+{
+    return k[min(max(b, 0), n - 1)];
+}
+__device__ inline int lcp_i(const uint64_t *__restrict__ k, int n, int i, uint64_t ki, int b)
+{
+    return lcp_kb(n, i, ki, b, key_at(k, n, b));
+}
+
+// Doubling probes of the range search: 2 << q for q < kExpProbes covers the
+// largest disk (2^21 bodies, beyond the splitter sort's 2^20).
+constexpr int kExpProbes = 21;
+
+// Last m in [0, span) with lcp(i, i + (base + m) d) > thr, a predicate that
+// holds at m = 0 and, once false, stays false.  Three bits of m per round:
+// the 7 probe keys of a round are read together (the scheduling barrier
+// keeps every load ahead of the first compare), so a search over 2^e
+// offsets costs ceil(e / 3) round trips instead of e.
+__device__ inline int last_true(const uint64_t *__restrict__ k, int n, int i, uint64_t ki, int d,
+                                int base, int thr, int span)
+{
+    int o = 0;
+    if (span <= 1) return 0;
+    for (int b = 31 - __clz(span - 1); b >= 0; b -= 3) {
+        const int low = max(b - 2, 0), ncand = (2 << (b - low)) - 1, step = 1 << low;
+        uint64_t kb[7];
+#pragma unroll
+        for (int c = 1; c <= 7; ++c) kb[c - 1] = key_at(k, n, i + (base + o + c * step) * d);
+        __builtin_amdgcn_sched_barrier(0);
+        int cnt = 0;  // the probes that hold come first
+#pragma unroll
+        for (int c = 1; c <= 7; ++c) {
+            const int m = o + c * step, pos = i + (base + m) * d;
+            cnt += ((c <= ncand) & (m < span) & (lcp_kb(n, i, ki, pos, kb[c - 1]) > thr)) ? 1 : 0;
+        }
+        o += cnt * step;
+    }
+    return o;
+}
+
 // Octree level of a binary node: floor(prefix bits / 3); identical keys
 // (prefix >= 64) form level-21 cells.  Key bit 63 is always zero.
 __device__ inline int oct_level(int plen) { return plen >= 64 ? kLevels : (plen - 1) / 3; }
@@ -1375,28 +1422,48 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     const int n = T.off[s + 1] - T.off[s];
     const uint64_t *k = keys + T.off[s];
     const int i = j - T.ioff[s];
-    const int d = (lcp(k, n, i, i + 1) - lcp(k, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = lcp(k, n, i, i - d);
+    const uint64_t ki = k[i];
+    const int lp = lcp_i(k, n, i, ki, i + 1), lm = lcp_i(k, n, i, ki, i - 1);
+    const int d = lp - lm >= 0 ? 1 : -1;
+    const int dmin = d > 0 ? lm : lp;
+    // The other end i + l d: l is the last m with lcp(i, i + m d) > dmin
+    // (true up to l, false beyond).  Every doubling probe 2, 4, ... is
+    // loaded at once; the last that holds, 2^q, bounds l in [2^q, 2^(q+1)).
     int lmax = 2;
-    while (lcp(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (lcp(k, n, i, i + (l + t) * d) > dmin) l += t;
+    {
+        uint64_t kp[kExpProbes];
+#pragma unroll
+        for (int q = 0; q < kExpProbes; ++q) kp[q] = key_at(k, n, i + (2 << q) * d);
+        __builtin_amdgcn_sched_barrier(0);
+        bool t[kExpProbes];
+#pragma unroll
+        for (int q = 0; q < kExpProbes; ++q)
+            t[q] = lcp_kb(n, i, ki, i + (2 << q) * d, kp[q]) > dmin;
+#pragma unroll
+        for (int q = 0; q < kExpProbes; ++q)
+            if (t[q]) lmax = 4 << q;  // the probes that hold come first
+        // (disks beyond 2^kExpProbes bodies: keep doubling one probe at a time)
+        while (t[kExpProbes - 1] && lcp_i(k, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
+    }
+    const int l = lmax / 2 + last_true(k, n, i, ki, d, lmax / 2, dmin, lmax / 2);
     const int jj = i + l * d;
-    const int dnode = lcp(k, n, i, jj);
-    int sp = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (lcp(k, n, i, i + (sp + t) * d) > dnode) sp += t;
-    } while (t > 1);
-    const int gamma = i + sp * d + min(d, 0);
     const int lo = min(i, jj), hi = max(i, jj);
+    // the far end's key and the keys beside the range, read together
+    const uint64_t kjj = key_at(k, n, jj), klo1 = key_at(k, n, lo - 1),
+                   khi1 = key_at(k, n, hi + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    const int dnode = lcp_kb(n, i, ki, jj, kjj);
+    // split: the last m in [0, l) with lcp(i, i + m d) > dnode
+    const int sp = last_true(k, n, i, ki, d, 0, dnode, l);
+    const int gamma = i + sp * d + min(d, 0);
     if (brange) brange[j] = make_int2(T.off[s] + lo, T.off[s] + hi);
     left[j] = lo == gamma ? ~(T.off[s] + gamma) : T.ioff[s] + gamma;
     right[j] = hi == gamma + 1 ? ~(T.off[s] + gamma + 1) : T.ioff[s] + gamma + 1;
     plen[j] = dnode;
     const int lev = oct_level(dnode);
-    const int dpar = max(lcp(k, n, lo - 1, lo), lcp(k, n, hi, hi + 1));
+    // parent prefix: max(lcp(lo - 1, lo), lcp(hi, hi + 1))
+    const uint64_t klo = lo == i ? ki : kjj, khi = hi == i ? ki : kjj;
+    const int dpar = max(lcp_kb(n, lo, klo, lo - 1, klo1), lcp_kb(n, hi, khi, hi + 1, khi1));
     octf[j] = (dpar < 0 || oct_level(dpar) != lev || lev == kLevels) ? 1 : 0;
     // mass and COM over sorted positions [lo, hi]
     const DVec<4> a = P[pbase(T, s) + lo], b = P[pbase(T, s) + hi + 1];
