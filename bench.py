@@ -394,14 +394,20 @@ def diff_window(nbx, np, torch, make_disk, args):
     torch.cuda.synchronize()
     fwd_ms = (time.perf_counter() - t0) * 1e3
     mask = annulus_mask(xT)
-    pm = torch.tensor(10.0, dtype=torch.float64, requires_grad=True)
-    rp = torch.tensor(5.2, dtype=torch.float64, requires_grad=True)
-    v0g = v0.clone().requires_grad_(True)
+
+    def forward_reverse():
+        pm = torch.tensor(10.0, dtype=torch.float64, requires_grad=True)
+        rp = torch.tensor(5.2, dtype=torch.float64, requires_grad=True)
+        v0g = v0.clone().requires_grad_(True)
+        xT, vT = window(x0, v0g, pm, rp, m, t, cfg)
+        L = window_loss(xT, vT, mask)
+        L.backward()
+        return pm, rp, v0g, L
+
+    forward_reverse()  # warm-up: reverse-mode buffers and the first launches
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xT, vT = window(x0, v0g, pm, rp, m, t, cfg)
-    L = window_loss(xT, vT, mask)
-    L.backward()
+    pm, rp, v0g, L = forward_reverse()
     torch.cuda.synchronize()
     fb_ms = (time.perf_counter() - t0) * 1e3
 
