@@ -9,6 +9,8 @@
 // Every phase is HBM-streaming integer/float work; nothing here is a GEMM.
 #include <float.h>
 
+#include <algorithm>
+
 #include "nbx_device.hpp"
 #include "nbx_launch.hpp"
 
@@ -1086,12 +1088,15 @@ __global__ __launch_bounds__(kSortThreads) void k_bucket_sort(
     const float4 *__restrict__ spay, const int *__restrict__ bm, uint64_t *__restrict__ kout,
     int *__restrict__ vout, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ z, const float *__restrict__ m, float4 *__restrict__ bodies,
-    uint64_t *__restrict__ spk, int *__restrict__ spi)
+    uint64_t *__restrict__ spk, int *__restrict__ spi, int nblk_max)
 {
     __shared__ uint64_t sk[kBucketCap];
     __shared__ int si[kBucketCap];
-    __shared__ int s_pre[kMaxSplitBlocks + 1];  // first bucket member of each contribution
-    __shared__ int s_src[kMaxSplitBlocks];      // staging position of each contribution
+    // per contribution (scatter block of the disk): the first bucket member
+    // and the staging position, sized at launch by the most scatter blocks
+    // of any disk of the tree (nblk_max + 1 and nblk_max ints)
+    extern __shared__ int s_dyn[];
+    int *s_pre = s_dyn, *s_src = s_dyn + nblk_max + 1;
     constexpr int CPT = kMaxSplitBlocks / kSortThreads;
     const int t = threadIdx.x;
     const int g = blockIdx.x;
@@ -1819,9 +1824,11 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
                            B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm, lpt);
     }
-    hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads), 0, st, T,
-                       B.keys[1], B.vals[1], B.spay, B.bm, B.keys[0], B.vals[0], x, y, z, mass,
-                       B.bodies, B.spk, B.spi);
+    int nblk_max = 1;
+    for (int q = 0; q < T.nseg; ++q) nblk_max = std::max(nblk_max, T.sblk[q + 1] - T.sblk[q]);
+    hipLaunchKernelGGL(k_bucket_sort, dim3(T.bkoff[T.nseg]), dim3(kSortThreads),
+                       sizeof(int) * (2 * nblk_max + 1), st, T, B.keys[1], B.vals[1], B.spay, B.bm,
+                       B.keys[0], B.vals[0], x, y, z, mass, B.bodies, B.spk, B.spi, nblk_max);
     B.cur = 0;
     B.split_ready = true;
 }
