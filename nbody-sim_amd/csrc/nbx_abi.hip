@@ -908,15 +908,20 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     if (e0 >= 0) ev_record(t, e0 + 2);
     // The collision pass reads the tree and the velocities and writes only
     // B.dv; the walk writes only the accelerations: without phase timing the
-    // collision pass runs on the side stream, filling the walk's idle slots.
+    // collision pass runs on the side stream beside the walk.  It is
+    // submitted first: when the walk has more waves than the GPU holds (a
+    // batch of disks), the collision waves get slots before the walk fills
+    // them, most end within microseconds, and the few long searches (bodies
+    // near the faces of large cells) finish under the walk instead of after
+    // it.
     const bool side_collide = sp->collisions && overlap;
-    if (side_collide) (void)hipEventRecord(t->fork, t->stream);
-    launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
-    if (side_collide) {  // submitted after the walk, so the walk's waves go first
+    if (side_collide) {
+        (void)hipEventRecord(t->fork, t->stream);
         (void)hipStreamWaitEvent(t->side, t->fork, 0);
         launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->side);
         (void)hipEventRecord(t->join, t->side);
     }
+    launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 3);
     if (side_collide)
         (void)hipStreamWaitEvent(t->stream, t->join, 0);
