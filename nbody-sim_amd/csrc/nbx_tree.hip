@@ -1590,6 +1590,8 @@ __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__r
 
 // Walk records: the children of octree node j into its block, the root
 // record of each disk at noff[s], and one-body disks (root = the body).
+// One thread per (node, child slot): the lanes of a node load its children's
+// data in one round trip and store its block as contiguous records.
 __global__ __launch_bounds__(kBlock) void k_oct_fill(
     SegTable T, const int *__restrict__ cnt, const int *__restrict__ first,
     const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
@@ -1597,7 +1599,8 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r,
     int *opar, int *opar_leaf, const int *__restrict__ kids)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = tid >> 3, k = tid & 7;
     // sorted-position range [lo, hi] covered by a record (adjoint only)
     auto put_r = [&](int code, int r) {
         if (rec_r) rec_r[r] = code < 0 ? make_int2(~code, ~code) : brange[code];
@@ -1624,7 +1627,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
             b = make_int4(__float_as_int(c.w), __float_as_int(c.w * c.w), first[code], cnt[code]);
         }
     };
-    if (j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
+    if (k == 0 && j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
         float4 a, c;
         int4 b;
         opar_leaf[T.off[j]] = -1;  // the body is the disk's root
@@ -1636,9 +1639,12 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         put_r(~T.off[j], T.noff[j]);
     }
     if (j >= T.ioff[T.nseg]) return;
-    if (cnt[j] == 0) return;
+    const int nk = cnt[j];
+    const int code = kids[8 * j + k];  // read beside the count (no dependence)
+    const int f0 = first[j];
+    if (nk == 0) return;
     const int s = seg_of_internal(T, j);
-    if (j == T.ioff[s]) {  // local node 0 is the disk's root
+    if (k == 0 && j == T.ioff[s]) {  // local node 0 is the disk's root
         float4 a, c;
         int4 b;
         opar[j] = -1;
@@ -1649,25 +1655,18 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         put_q(j, T.noff[s]);
         put_r(j, T.noff[s]);
     }
-    const int f0 = first[j], nk = cnt[j];
-    int code[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) code[k] = k < nk ? kids[8 * j + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (k >= nk) break;
-        float4 a, c;
-        int4 b;
-        // octree parent (binary index) of every record, for bottom-up searches
-        if (code[k] < 0) opar_leaf[~code[k]] = j;
-        else opar[code[k]] = j;
-        rec_of(code[k], a, b, c);
-        rec_a[f0 + k] = a;
-        rec_b[f0 + k] = b;
-        rec_c[f0 + k] = c;
-        put_q(code[k], f0 + k);
-        put_r(code[k], f0 + k);
-    }
+    if (k >= nk) return;
+    float4 a, c;
+    int4 b;
+    // octree parent (binary index) of every record, for bottom-up searches
+    if (code < 0) opar_leaf[~code] = j;
+    else opar[code] = j;
+    rec_of(code, a, b, c);
+    rec_a[f0 + k] = a;
+    rec_b[f0 + k] = b;
+    rec_c[f0 + k] = c;
+    put_q(code, f0 + k);
+    put_r(code, f0 + k);
 }
 
 // ===========================================================================
@@ -1863,7 +1862,7 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
     if (ni > 0)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
                            B.first, B.alloc, B.maxlev, B.err, B.kids);
-    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(ni > kMaxSeg ? ni : kMaxSeg), T, B.cnt, B.first,
+    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(8 * (ni > kMaxSeg ? ni : kMaxSeg)), T, B.cnt, B.first,
                        B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a, B.rec_b, B.rec_c,
                        B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar, B.opar_leaf, B.kids);
 }
