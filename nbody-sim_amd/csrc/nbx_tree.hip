@@ -1514,29 +1514,73 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     bin_cell[j] = make_float4(cx, cy, cz, hh);
 }
 
-// Enumerate the octree children of octree node j in Morton order: DFS over
-// the binary descendants folded into it (at most 7 inside one digit; bucket
-// nodes of identical keys are never folded).  f(k, code) per child.
-template <class F>
+// Enumerate the octree children of octree node j in Morton order: the
+// binary descendants folded into it (inside one digit; bucket nodes of
+// identical keys are never folded) are expanded level by level, each folded
+// entry replaced in place by its two children, so the list stays in Morton
+// order.  A digit leaves at most two folded levels below j's children, and
+// each level reads its fold flags, then the children of its folded entries,
+// in one round trip each.  Writes the codes to kids[0..n) and returns n, or
+// -1 when the folds do not end within a digit (an inconsistent tree).
 __device__ inline int oct_children(const int *__restrict__ left, const int *__restrict__ right,
-                                   const int *__restrict__ octf, int j, F f)
+                                   const int *__restrict__ octf, int j, int *kids)
 {
-    int stack[16];
-    int sp = 0, n = 0;
-    stack[sp++] = right[j];
-    stack[sp++] = left[j];
-    while (sp > 0) {
-        const int c = stack[--sp];
-        if (c < 0 || octf[c]) {
-            f(n, c);
-            ++n;
-        } else {
-            if (sp + 2 > 16) return -1;
-            stack[sp++] = right[c];
-            stack[sp++] = left[c];
+    int cur[8];
+    int n = 2;
+    cur[0] = left[j];
+    cur[1] = right[j];
+#pragma unroll
+    for (int lev = 0; lev < 3; ++lev) {
+        // every flag and child is read at a clamped index whatever the
+        // entry, so the loads of a level issue together
+        int of[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) of[q] = octf[max(q < n ? cur[q] : 0, 0)];
+        bool fold[8];
+        int nf = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            fold[q] = q < n && cur[q] >= 0 && !of[q];
+            nf += fold[q] ? 1 : 0;
         }
+        if (nf == 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (q < n) kids[q] = cur[q];
+            return n;
+        }
+        if (lev == 2 || n + nf > 8) return -1;
+        int lc[8], rc[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = max(q < n ? cur[q] : 0, 0);
+            lc[q] = left[c];
+            rc[q] = right[c];
+        }
+        // entry q moves to pos[q] (two slots when folded): a selection
+        // network with static indices keeps the lists in registers
+        int pos[8], m = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            pos[q] = m;
+            m += q < n ? (fold[q] ? 2 : 1) : 0;
+        }
+        int nxt[8];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            int v = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q < n && pos[q] == o) v = fold[q] ? lc[q] : cur[q];
+                if (q < n && fold[q] && pos[q] + 1 == o) v = rc[q];
+            }
+            nxt[o] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cur[q] = nxt[q];
+        n = m;
     }
-    return n;
+    return -1;
 }
 
 // Child counts and record blocks: one ticketed atomic per block and disk.
@@ -1563,9 +1607,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__r
     if (j < ni) {
         s = seg_of_internal(T, j);
         if (octf[j]) {
-            c = oct_children(left, right, octf, j, [&](int k, int code) {
-                if (k < 8) kids[8 * j + k] = code;
-            });
+            c = oct_children(left, right, octf, j, kids + 8 * j);
             if (c < 0 || c > 8) {
                 atomicOr(err, kErrTree);
                 c = 0;
