@@ -1379,10 +1379,6 @@ __device__ inline int lcp_i(const uint64_t *__restrict__ k, int n, int i, uint64
     return lcp_kb(n, i, ki, b, key_at(k, n, b));
 }
 
-// Doubling probes of the range search: 2 << q for q < kExpProbes covers the
-// largest disk (2^21 bodies, beyond the splitter sort's 2^20).
-constexpr int kExpProbes = 21;
-
 // Last m in [0, span) with lcp(i, i + (base + m) d) > thr, a predicate that
 // holds at m = 0 and, once false, stays false.  Three bits of m per round:
 // the 7 probe keys of a round are read together (the scheduling barrier
@@ -1414,7 +1410,7 @@ __device__ inline int last_true(const uint64_t *__restrict__ k, int n, int i, ui
 // (prefix >= 64) form level-21 cells.  Key bit 63 is always zero.
 __device__ inline int oct_level(int plen) { return plen >= 64 ? kLevels : (plen - 1) / 3; }
 
-__global__ __launch_bounds__(kBlock) void k_karras(
+__global__ __launch_bounds__(kBlock, 8) void k_karras(
     SegTable T, const uint64_t *__restrict__ keys, const DVec<4> *__restrict__ P,
     const DVec<6> *__restrict__ P2, const float4 *__restrict__ bodies,
     const Root *__restrict__ roots, int *left, int *right, int *plen, int *octf,
@@ -1432,23 +1428,24 @@ __global__ __launch_bounds__(kBlock) void k_karras(
     const int d = lp - lm >= 0 ? 1 : -1;
     const int dmin = d > 0 ? lm : lp;
     // The other end i + l d: l is the last m with lcp(i, i + m d) > dmin
-    // (true up to l, false beyond).  Every doubling probe 2, 4, ... is
-    // loaded at once; the last that holds, 2^q, bounds l in [2^q, 2^(q+1)).
+    // (true up to l, false beyond).  The doubling probes 2, 4, ... are read
+    // seven at a time (one round trip; most nodes need only the first
+    // group); the last that holds, 2^q, bounds l in [2^q, 2^(q+1)).
     int lmax = 2;
-    {
-        uint64_t kp[kExpProbes];
+    for (int q0 = 0;; q0 += 7) {
+        uint64_t kp[7];
 #pragma unroll
-        for (int q = 0; q < kExpProbes; ++q) kp[q] = key_at(k, n, i + (2 << q) * d);
+        for (int q = 0; q < 7; ++q) kp[q] = key_at(k, n, i + (2 << (q0 + q)) * d);
         __builtin_amdgcn_sched_barrier(0);
-        bool t[kExpProbes];
+        bool all = true;
 #pragma unroll
-        for (int q = 0; q < kExpProbes; ++q)
-            t[q] = lcp_kb(n, i, ki, i + (2 << q) * d, kp[q]) > dmin;
-#pragma unroll
-        for (int q = 0; q < kExpProbes; ++q)
-            if (t[q]) lmax = 4 << q;  // the probes that hold come first
-        // (disks beyond 2^kExpProbes bodies: keep doubling one probe at a time)
-        while (t[kExpProbes - 1] && lcp_i(k, n, i, ki, i + lmax * d) > dmin) lmax <<= 1;
+        for (int q = 0; q < 7; ++q) {
+            const bool tq = lcp_kb(n, i, ki, i + (2 << (q0 + q)) * d, kp[q]) > dmin;
+            if (tq && all) lmax = 4 << (q0 + q);  // the probes that hold come first
+            all = all && tq;
+        }
+        // (positions past the disk fail, so a disk's size bounds the groups)
+        if (!all || q0 + 7 >= 28) break;
     }
     const int l = lmax / 2 + last_true(k, n, i, ki, d, lmax / 2, dmin, lmax / 2);
     const int jj = i + l * d;
