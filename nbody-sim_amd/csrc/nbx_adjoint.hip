@@ -94,7 +94,7 @@ struct AdjVisitor {
             atomicAdd(G + 3 * r.id + 1, sy);
             atomicAdd(G + 3 * r.id + 2, sz);
         }
-        return open;
+        return !open;  // the walk opens the record where mine and not returned
     }
 };
 
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_adj_walk(
     float softening, float star_mass, const double *__restrict__ ab, double *xb, double *G,
     double *pg, int *err)
 {
-    __shared__ WaveLdsId lds[kWalkWaves];
+    __shared__ WaveLdsIdT<(TG > 32)> lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= M.wave[T.nseg]) return;  // uniform per wave

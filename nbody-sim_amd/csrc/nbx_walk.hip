@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
     float *saz, int *err, const int *__restrict__ order, int *__restrict__ cost, int lpt_stride)
 {
-    using Lds = typename std::conditional<QUAD, WaveLdsQ, WaveLds>::type;
+    using Lds = std::conditional_t<QUAD, WaveLdsQT<(TG > 32)>, WaveLdsT<(TG > 32)>>;
     __shared__ Lds lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int slot = blockIdx.x * kWalkWaves + wib;
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, int *n_acc,
     int *n_open, int *err, int *wstat)
 {
-    __shared__ WaveLds lds[kWalkWaves];
+    __shared__ WaveLdsT<(TG > 32)> lds[kWalkWaves];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= M.wave[T.nseg]) return;

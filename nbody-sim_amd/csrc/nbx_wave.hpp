@@ -34,9 +34,18 @@ constexpr int kBatchMaxSp = kGStack - kDeepMargin - 2 * kWave;
 static_assert(kBatchMaxSp >= 90, "stack too small for batched pops at theta = 0.5");
 constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
 
-struct StackEntry {
+// A stack entry: child block {first, count} and the mask of the targets that
+// opened its parent.  WIDE (64 targets per wave) keeps mask bits 32..63 in
+// mhi; waves of at most 32 targets use 12-byte entries.
+template <bool WIDE>
+struct StackEntryT {
     int first, count;
     unsigned mlo, mhi;
+};
+template <>
+struct StackEntryT<false> {
+    int first, count;
+    unsigned mlo;
 };
 
 // Staging slot of one popped record: {com, mass} and {mask bits 0..31, h^2,
@@ -49,29 +58,41 @@ struct StageSlot {
 };
 constexpr int kStageSlots = kWave + 8;
 
-struct WaveLds {
-    StackEntry st[kGStack];
+struct NoStageHi {};
+struct StageHi {
+    unsigned mhi[kStageSlots];  // mask bits 32..63 of the staged records
+};
+
+// LDS of one walking wave.  The footprint bounds the walk's occupancy (its
+// registers allow 8 waves per SIMD): 7,168 bytes with at most 32 targets per
+// wave (22 waves per CU), 8,992 bytes with 64.
+template <bool WIDE>
+struct WaveLdsT : std::conditional_t<WIDE, StageHi, NoStageHi> {
+    static constexpr bool kWide = WIDE;
+    StackEntryT<WIDE> st[kGStack];
     StageSlot s[kStageSlots];
-    unsigned mhi[kStageSlots];  // mask bits 32..63 (read with 64 targets per wave)
     int own[kWave];             // pop: flags of the lanes that start a stack entry
 };
 
-struct WaveLdsQ : WaveLds {
+template <bool WIDE>
+struct WaveLdsQT : WaveLdsT<WIDE> {
     float4 q[2 * kStageSlots];
 };
 
-// Squared distance of the opening test with every rounding fixed (no
-// contraction), so the force walk, its interaction counts and its adjoint
-// take identical decisions.
+// Squared distance of the opening test with every rounding fixed: one
+// product and two explicit fused multiply-adds, ((dx dx) + dy dy) + dz dz
+// in that order, so the force walk, its interaction counts and its adjoint
+// take identical decisions (the oracle's walk evaluates the same chain).
 __device__ inline float dist2(float dx, float dy, float dz)
 {
 #pragma clang fp contract(off)
-    return (dx * dx + dy * dy) + dz * dz;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
 // Staging with the record index of every slot (visitors that write per
 // record, such as the adjoint's node accumulators).
-struct WaveLdsId : WaveLds {
+template <bool WIDE>
+struct WaveLdsIdT : WaveLdsT<WIDE> {
     int id[kStageSlots];
 };
 
@@ -97,7 +118,7 @@ struct Rec {
 // owns one target and evaluates every record of the batch.
 // Visitor: visit_masked(rec, mine) on every lane; mine says whether the
 // lane's target owns the record.  The lane opens the (internal) record when
-// mine and the returned value are both true.
+// mine is true and the returned value (accepted) is false.
 // Returns the wave's work in record steps plus a per-batch overhead (the
 // cost that orders the waves of the next launch, longest first).
 template <int TG, bool QUAD, int U = 1, class Lds, class Visitor>
@@ -111,7 +132,17 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     const int tt = lane % TG, rr = lane / TG;
     // every lane writes the same entry: per-lane program order covers the
     // later reads, no cross-lane ordering is needed
-    L.st[0] = StackEntry{root, 1, (unsigned)active, (unsigned)(active >> 32)};
+    static_assert(TG <= 32 || Lds::kWide, "64 targets per wave need the wide stack entries");
+    using Entry = std::remove_reference_t<decltype(L.st[0])>;
+    auto entry = [](int first, int count, unsigned lo, unsigned hi) {
+        Entry e;
+        e.first = first;
+        e.count = count;
+        e.mlo = lo;
+        if constexpr (Lds::kWide) e.mhi = hi;
+        return e;
+    };
+    L.st[0] = entry(root, 1, (unsigned)active, (unsigned)(active >> 32));
     int sp = 1, sp_max = 1;
     int n_pop = 0, n_batch = 0, n_step = 0;  // records popped, batches, steps (wave-uniform)
     // Take up to 64 records from the top blocks (from the end of a block),
@@ -122,7 +153,7 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     auto pop = [&](int &nrec, int &my_rec, uint64_t &my_mask) {
         const bool deep = sp > kBatchMaxSp;
         const int ne = deep ? min(sp, 1) : min(sp, kWave);
-        StackEntry top{0, 0, 0u, 0u};
+        Entry top = entry(0, 0, 0u, 0u);
         if (lane < ne) top = L.st[sp - 1 - lane];
         const int incl = wave_incl_add(top.count);  // entries keep count >= 1
         const int excl = incl - top.count;
@@ -139,11 +170,12 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         my_rec = -1;
         my_mask = 0;
         if (lane < nrec) {
-            const StackEntry e = L.st[sp - 1 - j];
+            const Entry e = L.st[sp - 1 - j];
             const int take = min(e.count, kWave - ej);
             const int off = min(max(e.count - take + (lane - ej), 0), e.count - 1);
             my_rec = e.first + off;
-            my_mask = ((uint64_t)e.mhi << 32) | (uint64_t)e.mlo;
+            my_mask = (uint64_t)e.mlo;
+            if constexpr (Lds::kWide) my_mask |= (uint64_t)e.mhi << 32;
         }
         // (the partial entry is rewritten after every lane has read it)
         if (lane == used && lane < ne && excl < kWave) L.st[sp - 1 - lane].count = incl - kWave;
@@ -207,11 +239,14 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         };
         // a step covers SU = LS * U records: slot rr of unroll u takes record
         // k0 + u LS + rr, so every lane evaluates U independent records.
-        // Slots past the batch carry zero masks; the last prefetch reads past
-        // the batch and is discarded.
+        // Steps run in pairs with one exit test per pair (both halves leave
+        // their sums in the same registers, so the loop needs no copies).
+        // Slots past the batch (below 64) are staged with zero masks, so the
+        // odd step a pair may add evaluates nothing; the last prefetch reads
+        // past the batch and is discarded.
         constexpr int SU = LS * U;
-        static_assert(kWave % SU == 0 && kWave + SU <= kStageSlots,
-                      "the last prefetch stays inside the staging area");
+        static_assert(kWave % (2 * SU) == 0 && kWave + SU <= kStageSlots,
+                      "pairs of steps tile the batch; the last prefetch stays inside the staging area");
         // Open masks of the batch: lane k of om_r (bits 32..63 in om_h)
         // receives the mask of the targets that opened record k.  Steps only
         // record them; the children are pushed once, after the batch.
@@ -222,8 +257,9 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
                 const bool mine = TG > 32 ? ((mc[u] >> tt) & 1ull) != 0
                                           : (((unsigned)mc[u] >> tt) & 1u) != 0;
                 // (two ballots of plain compares: each is the compare's own
-                // lane mask, with no select in between)
-                const uint64_t om = ballot64(mine) & ballot64(vis.visit_masked(cur[u], mine));
+                // lane mask, with no select in between; the acceptance
+                // compare serves both the pull and the open mask)
+                const uint64_t om = ballot64(mine) & ~ballot64(vis.visit_masked(cur[u], mine));
 #pragma unroll
                 for (int q = 0; q < LS; ++q) {
                     // slot q of unroll u holds record k0 + u LS + q (< 64)
@@ -247,14 +283,12 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
             for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, cb[u], mb[u]);
             __builtin_amdgcn_sched_barrier(0);
             evaluate(ca, ma, k0);
-            ++n_step;
             k0 += SU;
-            if (k0 >= nrec) break;
 #pragma unroll
             for (int u = 0; u < U; ++u) stage(k0 + SU + u * LS + rr, ca[u], ma[u]);
             __builtin_amdgcn_sched_barrier(0);
             evaluate(cb, mb, k0);
-            ++n_step;
+            n_step += 2;
             k0 += SU;
             if (k0 >= nrec) break;
         }
@@ -267,7 +301,7 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
                 const int np = __popcll(fl);
                 const int at = sp + mbcnt64(fl);
                 if (opened && at < kGStack)
-                    L.st[at] = StackEntry{rf, rc, (unsigned)om_r, (unsigned)om_h};
+                    L.st[at] = entry(rf, rc, (unsigned)om_r, (unsigned)om_h);
                 if (sp + np > kGStack && lane == 0) atomicOr(err, kErrStack);
                 sp = min(sp + np, kGStack);
                 sp_max = max(sp_max, sp);
@@ -352,8 +386,8 @@ struct GravityVisitor {
             fy += g * dy - rinv5 * qy;
             fz += g * dz - rinv5 * qz;
         }
-        // the walk takes the open mask as ballot(mine) & ballot(result)
-        return !acc;
+        // the walk takes the open mask as ballot(mine) & ~ballot(acc)
+        return acc;
     }
 };
 
@@ -365,7 +399,7 @@ struct CountVisitor {
     int n_acc, n_open;
     __device__ inline bool visit_masked(const Rec &r, bool mine)
     {
-        return mine && visit(r, r.b.w == 0);
+        return !(mine && visit(r, r.b.w == 0));
     }
     __device__ inline bool visit(const Rec &r, bool leaf)
     {

@@ -726,6 +726,15 @@ typedef struct {
     double q[6];      /* traceless quadrupole about the COM: xx yy zz xy xz yz */
 } mt_node;
 
+/* Squared distance of the compressed-octree opening test:
+ * ((dx dx) + dy dy) + dz dz with each addition fused into its product,
+ * evaluated by fmaf (correctly rounded), so the CPU and GPU walks of the
+ * same tree take the same decisions. */
+static inline float mt_dist2(float dx, float dy, float dz)
+{
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
 typedef struct {
     const uint64_t *keys;
     const float *sx, *sy, *sz, *sm; /* bodies in sorted order */
@@ -884,7 +893,7 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
             {
                 const mt_node *r = &T.nodes[0];
                 float dx = r->cx - bx, dy = r->cy - by, dz = r->cz - bz;
-                float r2 = dx * dx + dy * dy + dz * dz;
+                float r2 = mt_dist2(dx, dy, dz);
                 if (r->h2 < theta2 * r2) {
                     float rinv = 1.0f / sqrtf(r2 + eps2);
                     float f = r->m * rinv * rinv * rinv;
@@ -911,7 +920,7 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
                         cm = T.nodes[c].m;
                     }
                     float dx = cxn - bx, dy = cyn - by, dz = czn - bz;
-                    float r2 = dx * dx + dy * dy + dz * dz;
+                    float r2 = mt_dist2(dx, dy, dz);
                     if (leaf || T.nodes[c].h2 < theta2 * r2) {
                         float rinv = 1.0f / sqrtf(r2 + eps2);
                         float f = cm * rinv * rinv * rinv;
@@ -987,7 +996,7 @@ long orc_mt_lists(const float *x, const float *y, const float *z, const float *m
             int top = 0;
             const mt_node *r = &T.nodes[0];
             float dx = r->cx - bx, dy = r->cy - by, dz = r->cz - bz;
-            if (r->h2 < theta2 * (dx * dx + dy * dy + dz * dz)) {
+            if (r->h2 < theta2 * mt_dist2(dx, dy, dz)) {
                 if (pass == 1) { lo[w] = r->lo; hi[w] = r->hi; ++w; }
                 ++c_i;
             } else {
@@ -1006,7 +1015,7 @@ long orc_mt_lists(const float *x, const float *y, const float *z, const float *m
                     }
                     const mt_node *nd = &T.nodes[c];
                     dx = nd->cx - bx; dy = nd->cy - by; dz = nd->cz - bz;
-                    if (nd->h2 < theta2 * (dx * dx + dy * dy + dz * dz)) {
+                    if (nd->h2 < theta2 * mt_dist2(dx, dy, dz)) {
                         if (pass == 1) { lo[w] = nd->lo; hi[w] = nd->hi; ++w; }
                         ++c_i;
                     } else if (top < 512) {
