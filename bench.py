@@ -304,10 +304,13 @@ def plain_driver_runs(nbx, steps):
 
 
 def batched_sweep(nbx, np, torch, make_disk, args):
-    """Six disks of uneven N, one perturber mass each, advanced together by
-    one replayed step (every phase one launch for all disks), against the six
-    disks run one after the other (a) by the C++ driver with the reference's
-    own call sequence and (b) as captured single-disk steps of this library."""
+    """Six disks of uneven N, one perturber mass each, advanced together:
+    two batched trees over contiguous disk groups (every phase one launch
+    per group), each replaying its captured step on its own stream, side by
+    side.  Against the six disks run one after the other (a) by the
+    C++ driver with the reference's own call sequence and (b) as captured
+    single-disk steps of this library; the six disks in one batched tree on
+    one stream are reported beside."""
     k = args.batch_steps
     disks = [make_disk(n, DISK_MASS, 0.5, 4.0, seed=100 + i) for i, n in enumerate(SWEEP_N)]
     sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
@@ -335,7 +338,31 @@ def batched_sweep(nbx, np, torch, make_disk, args):
         nbx.sync(t)
     torch.cuda.synchronize()
     seq_s = time.perf_counter() - t0
-    # one batched tree over all disks
+    # the six disks in two contiguous groups (even body totals), each one
+    # batched tree replaying its captured step on its own stream, side by side
+    groups = nbx.split_groups(SWEEP_N, 2)
+    members = []
+    for g in groups:
+        cat = {key: np.concatenate([disks[i][key] for i in g]) for key in disks[0]}
+        pg = nbx.to_gpu(cat)
+        tg = nbx.allocateOctreeBatch([SWEEP_N[i] for i in g])
+        dg = [dps[i] for i in g]
+        nbx.init_forces(tg, pg, sp, dg)
+        nbx.tune_walk(tg, 0.5, 1e-3, 1.0)
+        nbx.set_async(tg, True)
+        members.append((tg, pg, dg))
+    mt, mp, md = ([m[j] for m in members] for j in range(3))
+    nbx.step_concurrent(mt, mp, sp, md, 0, 5)
+    for tg in mt:
+        nbx.sync(tg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nbx.step_concurrent(mt, mp, sp, md, 5, k)
+    for tg in mt:
+        nbx.sync(tg)
+    torch.cuda.synchronize()
+    bat_s = time.perf_counter() - t0
+    # all six disks in one batched tree (one stream)
     cat = {key: np.concatenate([dd[key] for dd in disks]) for key in disks[0]}
     pb = nbx.to_gpu(cat)
     tb = nbx.allocateOctreeBatch(SWEEP_N)
@@ -349,16 +376,22 @@ def batched_sweep(nbx, np, torch, make_disk, args):
     nbx.step(tb, pb, sp, dps, 5, k)
     nbx.sync(tb)
     torch.cuda.synchronize()
-    bat_s = time.perf_counter() - t0
+    one_s = time.perf_counter() - t0
     total = float(sum(SWEEP_N))
     batched = total * k / bat_s
     runs = plain_driver_runs(nbx, args.plain_steps)
     plain_rate = sum(r["n"] * r["steps"] for r in runs) / sum(r["loop_s"] for r in runs)
     return {"disks": SWEEP_N, "perturber_mjup": SWEEP_MJUP, "steps": k,
-            "walk_shape_batched": nbx.walk_shape(tb),
+            "groups": groups,
+            "how": "two batched trees (contiguous disk groups), each replaying its "
+                   "captured step on its own stream, side by side (nbx_step_concurrent)",
+            "walk_shape_batched": [nbx.walk_shape(tg) for tg in mt],
             "walk_shape_captured_runs": [nbx.walk_shape(t) for _, t, _ in plain],
             "batched_particle_steps_per_s": batched,
             "batched_ms_per_step": bat_s / k * 1e3,
+            "single_tree": {"how": "all six disks in one batched tree on one stream",
+                            "particle_steps_per_s": total * k / one_s,
+                            "ms_per_step": one_s / k * 1e3},
             "plain_runs": {"how": "C++ driver, reference call sequence main.cu:85-138, "
                                   "one synchronous call per kernel",
                            "steps_per_disk": args.plain_steps,

@@ -169,6 +169,19 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
  * updatePerturber (Perturber.h:60-66) evaluated on the host. */
 int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
              const nbx_disk_params *dp, int32_t step0, int32_t n_steps);
+/* No reference counterpart (batching): n_steps steps of n_trees trees (each
+ * a disk or a batch of disks, with its own particles ps[i] and disk
+ * parameters dps[i]) stepped together: every tree replays its own captured
+ * step on its own stream, step by step, so one tree's build passes run
+ * beside another tree's force walk.  Each tree ends in exactly the state
+ * nbx_step gives it alone.  The group starts after the work already on
+ * trees[0]'s stream, and that stream waits for the whole group; without
+ * use_graph, or with phase timing on any tree, the trees are stepped one
+ * after the other.  A tree may appear once. */
+#define NBX_MAX_GROUP 8
+int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_t n_trees,
+                        const nbx_step_params *sp, const nbx_disk_params *const *dps,
+                        int32_t step0, int32_t n_steps);
 
 /* ---- walk shape and autotuner (no reference counterpart) ----------------- */
 /* Key sort algorithm: 0 = splitter buckets (sampled splitters, per-bucket

@@ -82,6 +82,9 @@ struct nbx_tree {
     int lpt_tg = 0;
     bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
     double *h_adj = nullptr;  // pinned copy of B.adj_p
+    // orders the streams of a group of trees stepped together
+    // (nbx_step_concurrent); never recorded inside a stream capture
+    hipEvent_t gord = nullptr;
 };
 
 namespace {
@@ -222,6 +225,7 @@ void free_bufs(nbx_tree *t)
     if (t->h_adj) (void)hipHostFree(t->h_adj);
     if (t->dtab_done) (void)hipEventDestroy(t->dtab_done);
     if (t->graph) (void)hipGraphExecDestroy(t->graph);
+    t->graph = nullptr;
     B = Bufs{};
     t->h_err = nullptr;
     t->h_dtab = nullptr;
@@ -296,10 +300,12 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
     t->own_stream = true;
     if ((e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&t->gord, hipEventDisableTiming)) != hipSuccess) {
         std::string msg = std::string("stream / event setup: ") + hipGetErrorString(e);
         if (t->side) (void)hipStreamDestroy(t->side);
         if (t->fork) (void)hipEventDestroy(t->fork);
+        if (t->join) (void)hipEventDestroy(t->join);
         (void)hipStreamDestroy(t->stream);
         delete t;
         return fail(NBX_E_HIP, "%s", msg.c_str());
@@ -459,6 +465,7 @@ int nbx_tree_destroy(nbx_tree *t)
     (void)hipStreamDestroy(t->side);
     (void)hipEventDestroy(t->fork);
     (void)hipEventDestroy(t->join);
+    (void)hipEventDestroy(t->gord);
     delete t;
     return NBX_OK;
 }
@@ -959,6 +966,45 @@ static std::vector<unsigned char> step_key(const nbx_particles *p, const nbx_ste
     return k;
 }
 
+// Per-step disk table of the next n_steps steps and the walk's wave-order
+// buffers, on the tree's own stream.
+static int prepare_steps(nbx_tree *t, const nbx_step_params *sp, const nbx_disk_params *dp,
+                         int32_t step0, int32_t n_steps, DiskTable &D)
+{
+    int rc = upload_steps(t, dp, step0, n_steps, sp->dt, false);
+    if (rc) return rc;
+    D = disk_rcoll(t, dp);
+    if (t->lpt && t->lpt_tg != t->walk_tg) {  // wave orders of another shape: start over
+        reset_lpt(t->B, t->stream);
+        t->lpt_tg = t->walk_tg;
+    }
+    return NBX_OK;
+}
+
+// The tree's captured step (kept while nothing it bakes in changes).
+static int ensure_step_graph(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                             const DiskTable &D)
+{
+    std::vector<unsigned char> key = step_key(p, sp, D, t->B, t->walk_tg);
+    if (t->graph && key == t->graph_key) return NBX_OK;
+    if (t->graph) {
+        HIPCHK(hipGraphExecDestroy(t->graph));
+        t->graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(t, p, sp, D, -1);
+    HIPCHK(hipStreamEndCapture(t->stream, &g));
+    hipError_t e = hipGraphInstantiate(&t->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        t->graph = nullptr;
+        return fail(NBX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    t->graph_key = key;
+    return NBX_OK;
+}
+
 int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
              const nbx_disk_params *dp, int32_t step0, int32_t n_steps)
 {
@@ -967,33 +1013,13 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
     if (n_steps < 0 || step0 < 0) return fail(NBX_E_ARG, "negative step range");
     if (n_steps == 0) return finish(t);
-    rc = upload_steps(t, dp, step0, n_steps, sp->dt, false);
+    DiskTable D;
+    rc = prepare_steps(t, sp, dp, step0, n_steps, D);
     if (rc) return rc;
-    const DiskTable D = disk_rcoll(t, dp);
-    if (t->lpt && t->lpt_tg != t->walk_tg) {  // wave orders of another shape: start over
-        reset_lpt(t->B, t->stream);
-        t->lpt_tg = t->walk_tg;
-    }
     const bool graph = sp->use_graph && !t->timing;
     if (graph) {
-        std::vector<unsigned char> key = step_key(p, sp, D, t->B, t->walk_tg);
-        if (!t->graph || key != t->graph_key) {
-            if (t->graph) {
-                HIPCHK(hipGraphExecDestroy(t->graph));
-                t->graph = nullptr;
-            }
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
-            enqueue_step(t, p, sp, D, -1);
-            HIPCHK(hipStreamEndCapture(t->stream, &g));
-            hipError_t e = hipGraphInstantiate(&t->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) {
-                t->graph = nullptr;
-                return fail(NBX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-            }
-            t->graph_key = key;
-        }
+        rc = ensure_step_graph(t, p, sp, D);
+        if (rc) return rc;
         for (int k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(t->graph, t->stream));
     } else {
         const int nev = 6 * n_steps;
@@ -1018,6 +1044,62 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     }
     t->stage = kSummarized;
     return finish(t);
+}
+
+// Several trees (each one disk or a batch of disks) stepped together: each
+// tree replays its own captured step (the graph nbx_step captures for it) on
+// its own stream, step by step, so the GPU runs one tree's latency-bound
+// build passes beside another tree's walk.  The trees' streams start after
+// the work already on trees[0]'s stream, and trees[0]'s stream waits for all
+// of them.  Every tree computes exactly what nbx_step computes for it alone.
+int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_t n_trees,
+                        const nbx_step_params *sp, const nbx_disk_params *const *dps,
+                        int32_t step0, int32_t n_steps)
+{
+    if (!trees || !ps || !dps || !sp) return fail(NBX_E_ARG, "null arguments");
+    if (n_trees < 1 || n_trees > NBX_MAX_GROUP)
+        return fail(NBX_E_ARG, "tree count %d not in [1,%d]", n_trees, NBX_MAX_GROUP);
+    if (n_steps < 0 || step0 < 0) return fail(NBX_E_ARG, "negative step range");
+    for (int i = 0; i < n_trees; ++i) {
+        int rc = check_tree_particles(trees[i], ps[i]);
+        if (rc) return rc;
+        if (!dps[i]) return fail(NBX_E_ARG, "null disk parameters of tree %d", i);
+        for (int j = 0; j < i; ++j)
+            if (trees[j] == trees[i]) return fail(NBX_E_ARG, "tree %d appears twice", i);
+    }
+    bool direct = !sp->use_graph;
+    for (int i = 0; i < n_trees; ++i) direct = direct || trees[i]->timing;
+    if (direct || n_steps == 0) {  // no graphs: each tree steps on its own stream in turn
+        for (int i = 0; i < n_trees; ++i) {
+            int rc = nbx_step(trees[i], ps[i], sp, dps[i], step0, n_steps);
+            if (rc) return rc;
+        }
+        return NBX_OK;
+    }
+    nbx_tree *lead = trees[0];
+    HIPCHK(hipEventRecord(lead->gord, lead->stream));
+    for (int i = 0; i < n_trees; ++i) {
+        nbx_tree *t = trees[i];
+        if (i > 0) HIPCHK(hipStreamWaitEvent(t->stream, lead->gord, 0));
+        DiskTable D;
+        int rc = prepare_steps(t, sp, dps[i], step0, n_steps, D);
+        if (rc) return rc;
+        rc = ensure_step_graph(t, ps[i], sp, D);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < n_steps; ++k)
+        for (int i = 0; i < n_trees; ++i)
+            HIPCHK(hipGraphLaunch(trees[i]->graph, trees[i]->stream));
+    for (int i = 1; i < n_trees; ++i) {
+        HIPCHK(hipEventRecord(trees[i]->gord, trees[i]->stream));
+        HIPCHK(hipStreamWaitEvent(lead->stream, trees[i]->gord, 0));
+    }
+    for (int i = 0; i < n_trees; ++i) {
+        trees[i]->stage = kSummarized;
+        int rc = finish(trees[i]);
+        if (rc) return rc;
+    }
+    return NBX_OK;
 }
 
 // ---------------------------------------------------------------------------

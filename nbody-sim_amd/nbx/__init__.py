@@ -105,7 +105,7 @@ EXPORTS = [
     "nbx_build", "nbx_summarize", "nbx_sort", "nbx_sort_idx", "nbx_sorted_keys", "nbx_force",
     "nbx_force_ex", "nbx_collide", "nbx_perturber_force", "nbx_half_kick", "nbx_drift",
     "nbx_reset_acceleration", "nbx_gas_drag", "nbx_fof", "nbx_init_forces", "nbx_step",
-    "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_probe_fp32", "nbx_walk_counts",
+    "nbx_step_concurrent", "nbx_tree_stats", "nbx_tree_set_timing", "nbx_phase_ms", "nbx_probe_fp32", "nbx_walk_counts",
     "nbx_tree_set_multipole", "nbx_tree_set_precision", "nbx_diagnostics",
     "nbx_tree_set_walk_shape", "nbx_tree_get_walk_shape", "nbx_tune_walk", "nbx_walk_stats",
     "nbx_tree_set_radix_bits", "nbx_tree_set_adjoint", "nbx_adj_force", "nbx_adj_step",
@@ -144,6 +144,8 @@ def lib():
             "nbx_gas_drag": [P, f32, f32, f32], "nbx_fof": [vp, P, f32, vp],
             "nbx_init_forces": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params)],
             "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
+            "nbx_step_concurrent": [C.POINTER(vp), C.POINTER(P), i32, C.POINTER(nbx_step_params),
+                                    C.POINTER(C.POINTER(nbx_disk_params)), i32, i32],
             "nbx_tree_stats": [vp, vp, vp], "nbx_tree_set_timing": [vp, i32],
             "nbx_phase_ms": [vp, C.c_char_p, C.POINTER(f32)],
             "nbx_probe_fp32": [i32, C.POINTER(C.c_double)],
@@ -565,6 +567,48 @@ def init_forces(t, p, sp, dps):
 def step(t, p, sp, dps, step0, n_steps):
     _check(lib().nbx_step(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps), step0, n_steps),
            "step")
+
+
+def step_concurrent(trees, ps, sp, dps_list, step0, n_steps):
+    """n_steps steps of several trees (each a disk or a batch of disks)
+    stepped together: every tree replays its captured step on its own
+    stream, so the trees' kernels run side by side; each tree ends as
+    nbx.step leaves it alone.  dps_list[i]: disk parameters of tree i (one
+    per disk of that tree)."""
+    n = len(trees)
+    if not (len(ps) == n == len(dps_list)):
+        raise ValueError("trees, particles and disk parameters differ in length")
+    hs = (C.c_void_p * n)(*[t.h for t in trees])
+    structs = [p.c() for p in ps]
+    pp = (C.POINTER(nbx_particles) * n)(*[C.pointer(c) for c in structs])
+    arrays = [_disk_array(d) for d in dps_list]
+    dd = (C.POINTER(nbx_disk_params) * n)(*[C.cast(a, C.POINTER(nbx_disk_params))
+                                             for a in arrays])
+    _check(lib().nbx_step_concurrent(hs, pp, n, C.byref(sp), dd, step0, n_steps),
+           "step_concurrent")
+
+
+def split_groups(counts, n_groups=2):
+    """Contiguous groups of disks for step_concurrent whose body totals are as
+    even as possible (the largest group total is minimal); returns lists of
+    disk indices."""
+    counts = [int(c) for c in counts]
+    k = max(1, min(n_groups, len(counts)))
+    best = None
+
+    def rec(start, left, acc):
+        nonlocal best
+        if left == 1:
+            cand = acc + [list(range(start, len(counts)))]
+            worst = max(sum(counts[i] for i in g) for g in cand)
+            if best is None or worst < best[0]:
+                best = (worst, cand)
+            return
+        for end in range(start + 1, len(counts) - left + 2):
+            rec(end, left - 1, acc + [list(range(start, end))])
+
+    rec(0, k, [])
+    return best[1]
 
 
 def set_multipole(t, order):

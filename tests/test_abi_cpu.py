@@ -49,6 +49,27 @@ def test_null_arguments_fail_before_device_work():
     assert L.nbx_tree_create_batch(C.byref(h), 17, too_many) == -1
     assert L.nbx_tree_destroy(None) == 0
     assert L.nbx_phase_ms(None, b"walk", None) == -1
+    assert L.nbx_step_concurrent(None, None, 1, None, None, 0, 1) == -1
+    assert b"null arguments" in L.nbx_last_error()
+
+
+def test_split_groups_contiguous_and_even():
+    """Contiguous disk groups with the smallest largest body total."""
+    sweep = [16384, 32768, 65536, 98304, 131072, 262144]
+    assert nbx.split_groups(sweep, 2) == [[0, 1, 2, 3, 4], [5]]
+    assert nbx.split_groups(sweep, 1) == [list(range(6))]
+    assert nbx.split_groups([5, 5, 5, 5], 2) == [[0, 1], [2, 3]]
+    assert nbx.split_groups([7], 3) == [[0]]
+    assert nbx.split_groups([1, 2, 3], 5) == [[0], [1], [2]]
+    # brute force over every contiguous split of a ragged list
+    counts = [9, 1, 4, 7, 2, 8, 3]
+    for k in (2, 3):
+        g = nbx.split_groups(counts, k)
+        assert sum(g, []) == list(range(len(counts))) and len(g) == k
+        import itertools
+        best = min(max(sum(counts[a:b]) for a, b in zip((0,) + c, c + (len(counts),)))
+                   for c in itertools.combinations(range(1, len(counts)), k - 1))
+        assert max(sum(counts[i] for i in x) for x in g) == best
 
 
 def test_missing_library_fails_loudly(monkeypatch):

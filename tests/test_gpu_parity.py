@@ -472,6 +472,57 @@ def test_batched_members_equal_plain_runs(gpu):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_concurrent_groups_equal_separate_steps(gpu, use_graph):
+    """Two batched trees stepped together (each replaying its captured step
+    on its own stream; with use_graph off, one after the other) end in
+    exactly the states each reaches alone with nbx.step; three calls, so the
+    step graphs are captured, recaptured when the per-step table grows, and
+    replayed with a new step0; a tree may not appear twice in a group."""
+    import torch
+    nbx = gpu
+    counts = [3000, 1024, 5000]
+    masses = [0.0, 1.0, 10.0]
+    groups = [[0, 1], [2]]
+    shapes = [32, 16]
+    disks = [disk(n, mass=1e-5, seed=40 + i) for i, n in enumerate(counts)]
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True, use_graph=use_graph)
+    dps = []
+    for n, mj in zip(counts, masses):
+        r_coll = nbx.particlePhysicalRadius(np.float32(1e-5 / n))
+        dps.append(nbx.disk_params(r_coll, nbx.makePerturber(mj > 0, mj, 5.2)))
+
+    def make(g, shape):
+        cat = {k: np.concatenate([disks[i][k] for i in g]) for k in disks[0]}
+        p = nbx.to_gpu(cat)
+        t = nbx.allocateOctreeBatch([counts[i] for i in g])
+        dd = [dps[i] for i in g]
+        nbx.init_forces(t, p, sp, dd)
+        nbx.set_walk_shape(t, shape)
+        return t, p, dd
+
+    alone = []
+    for g, shape in zip(groups, shapes):
+        t, p, dd = make(g, shape)
+        nbx.step(t, p, sp, dd, 0, 3)
+        nbx.step(t, p, sp, dd, 3, 70)
+        nbx.step(t, p, sp, dd, 73, 2)
+        alone.append(p.numpy())
+    members = [make(g, shape) for g, shape in zip(groups, shapes)]
+    ts, ps, ds = ([m[j] for m in members] for j in range(3))
+    nbx.step_concurrent(ts, ps, sp, ds, 0, 3)
+    nbx.step_concurrent(ts, ps, sp, ds, 3, 70)
+    nbx.step_concurrent(ts, ps, sp, ds, 73, 2)
+    for p, ref in zip(ps, alone):
+        g = p.numpy()
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
+            np.testing.assert_array_equal(g[k], ref[k], err_msg=k)
+    with pytest.raises(RuntimeError, match="appears twice"):
+        nbx.step_concurrent([ts[0], ts[0]], [ps[0], ps[0]], sp, [ds[0], ds[0]], 75, 1)
+    torch.cuda.synchronize()
+
+
 def test_argument_errors_raise(gpu):
     nbx = gpu
     d = disk(100)
