@@ -82,6 +82,7 @@ struct nbx_tree {
     int lpt_tg = 0;
     bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
     double *h_adj = nullptr;  // pinned copy of B.adj_p
+    int cus = 0;  // compute units of the device the tree was created on
     // orders the streams of a group of trees stepped together
     // (nbx_step_concurrent); never recorded inside a stream capture
     hipEvent_t gord = nullptr;
@@ -292,10 +293,13 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
         T.sblk[s] = T.sblk[nseg];
         T.moff[s] = T.moff[nseg];
     }
-    hipError_t e = hipStreamCreate(&t->stream);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipStreamCreate(&t->stream);
     if (e != hipSuccess) {
         delete t;
-        return fail(NBX_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        return fail(NBX_E_HIP, "stream setup: %s", hipGetErrorString(e));
     }
     t->own_stream = true;
     if ((e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking)) != hipSuccess ||
@@ -921,14 +925,24 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     // them, most end within microseconds, and the few long searches (bodies
     // near the faces of large cells) finish under the walk instead of after
     // it.
+    // Which of the two is submitted first decides which one a replayed graph
+    // keeps on the launch queue.  When every wave of the walk is resident at
+    // once (at most 16 waves per compute unit), the walk goes first: it stays
+    // on the launch queue and the step pays no cross-queue hand-off before
+    // or after it.  A larger walk (a batch of disks) is run in several
+    // rounds of waves, and the collision pass goes first so its waves get
+    // slots before the walk fills them.
     const bool side_collide = sp->collisions && overlap;
-    if (side_collide) {
-        (void)hipEventRecord(t->fork, t->stream);
+    const bool walk_first = walk_waves(t->T, t->walk_tg) <= 16 * t->cus;
+    auto side_pass = [&]() {
         (void)hipStreamWaitEvent(t->side, t->fork, 0);
         launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->side);
         (void)hipEventRecord(t->join, t->side);
-    }
+    };
+    if (side_collide) (void)hipEventRecord(t->fork, t->stream);
+    if (side_collide && !walk_first) side_pass();
     launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
+    if (side_collide && walk_first) side_pass();
     if (e0 >= 0) ev_record(t, e0 + 3);
     if (side_collide)
         (void)hipStreamWaitEvent(t->stream, t->join, 0);
