@@ -472,13 +472,14 @@ def test_batched_members_equal_plain_runs(gpu):
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_concurrent_groups_equal_separate_steps(gpu, use_graph):
+@pytest.mark.parametrize("use_graph,asynchronous", [(True, False), (False, False), (True, True)])
+def test_concurrent_groups_equal_separate_steps(gpu, use_graph, asynchronous):
     """Two batched trees stepped together (each replaying its captured step
     on its own stream; with use_graph off, one after the other) end in
     exactly the states each reaches alone with nbx.step; three calls, so the
     step graphs are captured, recaptured when the per-step table grows, and
-    replayed with a new step0; a tree may not appear twice in a group."""
+    replayed with a new step0; a tree may not appear twice in a group.  In
+    asynchronous mode nothing waits until the trees are synced at the end."""
     import torch
     nbx = gpu
     counts = [3000, 1024, 5000]
@@ -511,9 +512,14 @@ def test_concurrent_groups_equal_separate_steps(gpu, use_graph):
         alone.append(p.numpy())
     members = [make(g, shape) for g, shape in zip(groups, shapes)]
     ts, ps, ds = ([m[j] for m in members] for j in range(3))
+    for t in ts:
+        nbx.set_async(t, asynchronous)
     nbx.step_concurrent(ts, ps, sp, ds, 0, 3)
     nbx.step_concurrent(ts, ps, sp, ds, 3, 70)
     nbx.step_concurrent(ts, ps, sp, ds, 73, 2)
+    for t in ts:
+        nbx.sync(t)
+        nbx.set_async(t, False)
     for p, ref in zip(ps, alone):
         g = p.numpy()
         for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"):
