@@ -593,7 +593,10 @@ def split_groups(counts, n_groups=2):
     even as possible (the largest group total is minimal); returns lists of
     disk indices."""
     counts = [int(c) for c in counts]
-    k = max(1, min(n_groups, len(counts)))
+    if not counts or n_groups < 1 or min(counts) <= 0:
+        raise ValueError("split_groups needs at least one disk, positive body counts "
+                         "and n_groups >= 1")
+    k = min(n_groups, len(counts))
     best = None
 
     def rec(start, left, acc):

@@ -61,6 +61,9 @@ def test_split_groups_contiguous_and_even():
     assert nbx.split_groups([5, 5, 5, 5], 2) == [[0, 1], [2, 3]]
     assert nbx.split_groups([7], 3) == [[0]]
     assert nbx.split_groups([1, 2, 3], 5) == [[0], [1], [2]]
+    for bad in (([], 2), ([4, 0], 2), ([3], 0)):
+        with pytest.raises(ValueError):
+            nbx.split_groups(*bad)
     # brute force over every contiguous split of a ragged list
     counts = [9, 1, 4, 7, 2, 8, 3]
     for k in (2, 3):
