@@ -1081,6 +1081,9 @@ int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_
         for (int j = 0; j < i; ++j)
             if (trees[j] == trees[i]) return fail(NBX_E_ARG, "tree %d appears twice", i);
     }
+    nbx_tree *lead = trees[0];
+    HIPCHK(hipEventRecord(lead->gord, lead->stream));
+    for (int i = 1; i < n_trees; ++i) HIPCHK(hipStreamWaitEvent(trees[i]->stream, lead->gord, 0));
     bool direct = !sp->use_graph;
     for (int i = 0; i < n_trees; ++i) direct = direct || trees[i]->timing;
     if (direct || n_steps == 0) {  // no graphs: each tree steps on its own stream in turn
@@ -1088,13 +1091,14 @@ int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_
             int rc = nbx_step(trees[i], ps[i], sp, dps[i], step0, n_steps);
             if (rc) return rc;
         }
+        for (int i = 1; i < n_trees; ++i) {
+            HIPCHK(hipEventRecord(trees[i]->gord, trees[i]->stream));
+            HIPCHK(hipStreamWaitEvent(lead->stream, trees[i]->gord, 0));
+        }
         return NBX_OK;
     }
-    nbx_tree *lead = trees[0];
-    HIPCHK(hipEventRecord(lead->gord, lead->stream));
     for (int i = 0; i < n_trees; ++i) {
         nbx_tree *t = trees[i];
-        if (i > 0) HIPCHK(hipStreamWaitEvent(t->stream, lead->gord, 0));
         DiskTable D;
         int rc = prepare_steps(t, sp, dps[i], step0, n_steps, D);
         if (rc) return rc;
