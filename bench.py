@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU baseline sample length in steps (0: about 15 s)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-sweep leg")
-    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--plain-steps", type=int, default=40,
                     help="steps per disk of the reference-call-sequence sweep runs")
     ap.add_argument("--no-graph", action="store_true",
