@@ -75,7 +75,9 @@ int nbx_tree_destroy(nbx_tree *t);
 /* resetOctree (BHTree.cu:397-420): clears the collision deltas; the Morton
  * build needs no pool reset. */
 int nbx_tree_reset(nbx_tree *t);
-/* Stream control: `stream` is a hipStream_t (NULL = the tree's own stream).
+/* Stream control: `stream` is a hipStream_t (NULL = the tree's own stream,
+ * which the tree creates and keeps until nbx_tree_destroy; a caller's
+ * stream is never destroyed by the tree).
  * async=1 makes entry points return without waiting for the device. */
 int nbx_tree_set_stream(nbx_tree *t, void *stream);
 /* No reference counterpart (accuracy option): multipole order of the tree
@@ -154,7 +156,11 @@ typedef struct nbx_step_params {
     int32_t use_graph;       /* 1 = replay one captured hipGraph per step         */
 } nbx_step_params;
 
-/* Per-disk values (one per disk of a batched tree, one for a plain tree). */
+/* Per-disk values.  Every entry point taking `const nbx_disk_params *dp`
+ * reads exactly one entry per disk of the tree: dp must point to n_seg
+ * entries for a tree of nbx_tree_create_batch (n_seg disks), one for a
+ * plain tree.  The length cannot be checked through a pointer; a shorter
+ * array is read past its end. */
 typedef struct nbx_disk_params {
     float r_coll;            /* collision_radius_factor * particlePhysicalRadius */
     int32_t perturber_enabled;

@@ -13,7 +13,6 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "nbx.h"
@@ -24,6 +23,9 @@ inline void check(int rc, const char *op)
 {
     if (rc != NBX_OK) throw std::runtime_error(std::string(op) + ": " + nbx_last_error());
 }
+
+// Config (Config.h:5-35).
+using Config = nbx_config;
 
 // ParticleData (ParticleData.h:19-32): device pointers when allocated by
 // allocateParticlesGPU, host pointers when allocated by allocateParticlesCPU.
@@ -161,50 +163,99 @@ struct Aggregate {
     std::vector<int> particle_ids;
 };
 
-// `gpu` holds device pointers, `host` a host copy of the same state (the
-// reference calls detectAggregates after copyToCPU, main.cu:144-148).
-inline std::vector<Aggregate> detectAggregates(OctreeData &t, const ParticleData &gpu,
-                                               const ParticleData &host, float link_length,
-                                               float particle_mass_msun)
+// Aggregates of a partition: groups of >= 2 bodies (lab[i] = smallest index
+// of i's group, host array), each with the float sums of Aggregate.cpp:58-69
+// over its members in ascending index, from the host state; aggregates are
+// numbered in order of their smallest member.
+inline std::vector<Aggregate> aggregates_of_labels(const ParticleData &host, const int32_t *lab,
+                                                   float particle_mass_msun)
 {
-    const int n = gpu.n;
-    // device label buffer: the id array of a scratch ParticleData
-    ParticleData lab{};
-    check(nbx_particles_alloc(&lab, n), "detectAggregates labels");
-    check(nbx_fof(t.h, &gpu, link_length, lab.id), "detectAggregates");
-    ParticleData lab_host = allocateParticlesCPU(n);
-    check(nbx_copy_to_host(&lab, &lab_host), "detectAggregates labels");
-    std::unordered_map<int, std::vector<int>> groups;
-    for (int i = 0; i < n; ++i) groups[lab_host.id[i]].push_back(i);
-    freeParticlesCPU(lab_host);
-    check(nbx_particles_free(&lab), "detectAggregates labels");
-    std::vector<int> roots;
-    for (auto &g : groups)
-        if (g.second.size() >= 2) roots.push_back(g.first);
-    std::sort(roots.begin(), roots.end());
+    const int n = host.n;
+    std::vector<int> size(n, 0);
+    for (int i = 0; i < n; ++i) ++size[lab[i]];
     std::vector<Aggregate> out;
-    for (int r : roots) {
-        const std::vector<int> &m = groups[r];
-        Aggregate a;
-        a.id = (int)out.size();
-        a.n_particles = (int)m.size();
-        a.particle_ids = m;
-        a.mass_msun = a.n_particles * particle_mass_msun;
-        for (int idx : m) {
-            a.cx += host.x[idx]; a.cy += host.y[idx]; a.cz += host.z[idx];
-            a.vx += host.vx[idx]; a.vy += host.vy[idx]; a.vz += host.vz[idx];
+    std::vector<int> slot(n, -1);
+    for (int i = 0; i < n; ++i) {  // roots are the smallest members: ascending order
+        const int r = lab[i];
+        if (size[r] < 2) continue;
+        if (slot[r] < 0) {
+            slot[r] = (int)out.size();
+            Aggregate a;
+            a.id = slot[r];
+            a.n_particles = size[r];
+            a.mass_msun = a.n_particles * particle_mass_msun;
+            a.particle_ids.reserve(size[r]);
+            out.push_back(std::move(a));
         }
+        Aggregate &a = out[slot[r]];
+        a.particle_ids.push_back(i);
+        a.cx += host.x[i]; a.cy += host.y[i]; a.cz += host.z[i];
+        a.vx += host.vx[i]; a.vy += host.vy[i]; a.vz += host.vz[i];
+    }
+    for (Aggregate &a : out) {
         float inv = 1.0f / static_cast<float>(a.n_particles);
         a.cx *= inv; a.cy *= inv; a.cz *= inv;
         a.vx *= inv; a.vy *= inv; a.vz *= inv;
         a.dist_center = std::sqrt(a.cx * a.cx + a.cy * a.cy + a.cz * a.cz);
-        out.push_back(std::move(a));
     }
     return out;
 }
 
+// `gpu` holds device pointers, `host` a host copy of the same state (the
+// reference calls detectAggregates after copyToCPU, main.cu:144-148): the
+// partition uses the caller's tree scratch.
+inline std::vector<Aggregate> detectAggregates(OctreeData &t, const ParticleData &gpu,
+                                               const ParticleData &host, float link_length,
+                                               float particle_mass_msun)
+{
+    // device label buffer: the id array of a scratch ParticleData
+    ParticleData lab{}, lab_host{};
+    try {
+        check(nbx_particles_alloc(&lab, gpu.n), "detectAggregates labels");
+        check(nbx_fof(t.h, &gpu, link_length, lab.id), "detectAggregates");
+        lab_host = allocateParticlesCPU(gpu.n);
+        check(nbx_copy_to_host(&lab, &lab_host), "detectAggregates labels");
+        std::vector<Aggregate> out = aggregates_of_labels(host, lab_host.id, particle_mass_msun);
+        freeParticlesCPU(lab_host);
+        check(nbx_particles_free(&lab), "detectAggregates labels");
+        return out;
+    } catch (...) {
+        freeParticlesCPU(lab_host);
+        nbx_particles_free(&lab);
+        throw;
+    }
+}
+
+// detectAggregates (Aggregate.h:17-18) with the reference's own signature:
+// `p` is a HOST ParticleData, as at main.cu:148 after copyToCPU.  The
+// positions go to the device, where a tree of their own gives the
+// partition (the reference's O(N^2) union-find, Aggregate.cpp:26-76, has
+// the same components); the sums use cfg.particle_mass_msun.
+inline std::vector<Aggregate> detectAggregates(const ParticleData &p, float link_length,
+                                               const Config &cfg)
+{
+    ParticleData dev{};
+    nbx_tree *h = nullptr;
+    try {
+        check(nbx_particles_alloc(&dev, p.n), "detectAggregates particles");
+        check(nbx_copy_to_device(&p, &dev), "detectAggregates particles");
+        check(nbx_tree_create(&h, p.n), "detectAggregates tree");
+        OctreeData t;
+        t.h = h;
+        t.n_bodies = p.n;
+        std::vector<Aggregate> out = detectAggregates(t, dev, p, link_length,
+                                                      cfg.particle_mass_msun);
+        check(nbx_tree_destroy(h), "detectAggregates tree");
+        check(nbx_particles_free(&dev), "detectAggregates particles");
+        return out;
+    } catch (...) {
+        if (h) nbx_tree_destroy(h);
+        nbx_particles_free(&dev);
+        throw;
+    }
+}
+
 // ---- run support (Config.h, DiskInit.h, OutputWriter.h) -------------------
-using Config = nbx_config;
 
 inline Config loadConfig(const std::string &path)  // Config.cpp:5-55
 {

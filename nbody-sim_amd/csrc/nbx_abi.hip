@@ -55,8 +55,8 @@ void set_last_error(const char *msg) { g_err = msg; }
 struct nbx_tree {
     SegTable T{};
     Bufs B{};
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
+    hipStream_t stream = nullptr;  // stream every entry point enqueues on
+    hipStream_t owned = nullptr;   // the stream the tree created (destroyed with it)
     bool async = false;
     int stage = kNone;
     int *h_err = nullptr;  // pinned copy of B.err
@@ -132,67 +132,84 @@ int dmalloc(T **p, size_t count)
     return NBX_OK;
 }
 
-int alloc_bufs(nbx_tree *t)
+// The scratch of the tree build, summarize, walks and searches (every buffer
+// the FoF path touches), in one fixed order: take(ptr, count) assigns each.
+template <class Take>
+void core_layout(const SegTable &T, Bufs &B, Take &&take)
 {
-    Bufs &B = t->B;
-    const int n = t->T.n_total;
+    const int n = T.n_total;
     B.n = n;
     B.radix_bits = 0;
-    for (int s = 0; s < t->T.nseg; ++s)
-        if (t->T.off[s + 1] - t->T.off[s] > kSplitMaxN) B.radix_bits = 8;
+    for (int s = 0; s < T.nseg; ++s)
+        if (T.off[s + 1] - T.off[s] > kSplitMaxN) B.radix_bits = 8;
     B.n_sweep_tiles = (n + kSweepTile - 1) / kSweepTile;
-    B.n_scan_tiles = t->T.stile[t->T.nseg];
-    int rc = NBX_OK;
-#define ALLOC(ptr, cnt) \
-    if ((rc = dmalloc(&(ptr), (size_t)(cnt))) != NBX_OK) return rc
-    ALLOC(B.keys[0], n);
-    ALLOC(B.keys[1], n);
-    ALLOC(B.vals[0], n);
-    ALLOC(B.vals[1], n);
-    ALLOC(B.ghist, kHistInts);
-    ALLOC(B.spk, t->T.bkoff[t->T.nseg]);
-    ALLOC(B.spi, t->T.bkoff[t->T.nseg]);
-    ALLOC(B.bm, t->T.moff[t->T.nseg]);
-    ALLOC(B.spay, n);
-    B.lpt_stride = walk_waves(t->T, 8);
-    ALLOC(B.lpt_order, 2 * B.lpt_stride);
-    ALLOC(B.lpt_cost, 2 * B.lpt_stride);
-    ALLOC(B.lpt_hist, (size_t)lpt_parts(B.lpt_stride) * kLptBins);
-    ALLOC(B.status[0], 2048 * B.n_sweep_tiles);
-    ALLOC(B.status[1], 2048 * B.n_sweep_tiles);
-    ALLOC(B.tickets, 32);
-    ALLOC(B.prefix, n + t->T.nseg);
+    B.n_scan_tiles = T.stile[T.nseg];
+    B.n_rec = T.noff[T.nseg] + kRecPad;
+    take(B.keys[0], n);
+    take(B.keys[1], n);
+    take(B.vals[0], n);
+    take(B.vals[1], n);
+    take(B.ghist, kHistInts);
+    take(B.spk, T.bkoff[T.nseg]);
+    take(B.spi, T.bkoff[T.nseg]);
+    take(B.bm, T.moff[T.nseg]);
+    take(B.spay, n);
+    take(B.tickets, 32);
+    take(B.prefix, n + T.nseg);
     // scan look-back words of both prefix scans: 4 + 4 + 6 + 6 doubles per tile
-    ALLOC(B.scanw, (size_t)kScanWords * B.n_scan_tiles);
+    take(B.scanw, (size_t)kScanWords * B.n_scan_tiles);
+    take(B.bodies, n);
+    take(B.left, n);
+    take(B.right, n);
+    take(B.plen, n);
+    take(B.oct, n);
+    take(B.cnt, n);
+    take(B.first, n);
+    take(B.alloc, 2 * kMaxSeg);
+    take(B.opar, n);
+    take(B.opar_leaf, n);
+    take(B.kids, 8 * (size_t)n);
+    take(B.bin_com, n);
+    take(B.bin_cell, n);
+    take(B.rec_a, B.n_rec);
+    take(B.rec_b, B.n_rec);
+    take(B.rec_c, B.n_rec);
+    take(B.partial, 6 * T.blk[T.nseg]);
+    take(B.roots, kMaxSeg);
+    take(B.err, 4);
+    take(B.labels, n);
+}
+
+// Views into the look-back words (after the layout assigned scanw).
+void scan_views(Bufs &B)
+{
     B.pagg = (DVec<4> *)B.scanw;
     B.pinc = B.pagg + B.n_scan_tiles;
     B.pagg2 = (DVec<6> *)(B.scanw + 8 * (size_t)B.n_scan_tiles);
     B.pinc2 = B.pagg2 + B.n_scan_tiles;
-    ALLOC(B.bodies, n);
-    ALLOC(B.left, n);
-    ALLOC(B.right, n);
-    ALLOC(B.plen, n);
-    ALLOC(B.oct, n);
-    ALLOC(B.cnt, n);
-    ALLOC(B.first, n);
-    ALLOC(B.alloc, 2 * kMaxSeg);
     B.maxlev = B.alloc + kMaxSeg;  // adjacent: cleared by one fill
-    ALLOC(B.opar, n);
-    ALLOC(B.opar_leaf, n);
-    ALLOC(B.kids, 8 * (size_t)n);
-    ALLOC(B.bin_com, n);
-    ALLOC(B.bin_cell, n);
-    B.n_rec = t->T.noff[t->T.nseg] + kRecPad;
-    ALLOC(B.rec_a, B.n_rec);
-    ALLOC(B.rec_b, B.n_rec);
-    ALLOC(B.rec_c, B.n_rec);
-    ALLOC(B.partial, 6 * t->T.blk[t->T.nseg]);
-    ALLOC(B.roots, kMaxSeg);
-    ALLOC(B.dv, 3 * n);
-    ALLOC(B.err, 4);
-    ALLOC(B.labels, n);
-    ALLOC(B.step_ctr, 1);
-#undef ALLOC
+}
+
+int alloc_bufs(nbx_tree *t)
+{
+    Bufs &B = t->B;
+    const int n = t->T.n_total;
+    int rc = NBX_OK;
+    auto take = [&rc](auto *&ptr, size_t count) {
+        if (rc == NBX_OK) rc = dmalloc(&ptr, count);
+    };
+    core_layout(t->T, B, take);
+    if (rc != NBX_OK) return rc;
+    scan_views(B);
+    B.lpt_stride = walk_waves(t->T, 8);
+    take(B.lpt_order, 2 * (size_t)B.lpt_stride);
+    take(B.lpt_cost, 2 * (size_t)B.lpt_stride);
+    take(B.lpt_hist, (size_t)lpt_parts(B.lpt_stride) * kLptBins);
+    take(B.status[0], 2048 * (size_t)B.n_sweep_tiles);
+    take(B.status[1], 2048 * (size_t)B.n_sweep_tiles);
+    take(B.dv, 3 * (size_t)n);
+    take(B.step_ctr, 1);
+    if (rc != NBX_OK) return rc;
     HIPCHK(hipMemset(B.rec_a, 0, sizeof(float4) * B.n_rec));
     HIPCHK(hipMemset(B.rec_b, 0, sizeof(int4) * B.n_rec));
     HIPCHK(hipMemset(B.rec_c, 0, sizeof(float4) * B.n_rec));
@@ -243,32 +260,36 @@ int check_tree_particles(nbx_tree *t, const nbx_particles *p)
     return NBX_OK;
 }
 
-int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
+// Streams and events of a tree (whichever were created), then the tree.
+void destroy_handles(nbx_tree *t)
 {
-    if (!out) return fail(NBX_E_ARG, "null output pointer");
-    *out = nullptr;
+    for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
+    t->ev.clear();
+    if (t->owned) (void)hipStreamDestroy(t->owned);
+    if (t->side) (void)hipStreamDestroy(t->side);
+    if (t->fork) (void)hipEventDestroy(t->fork);
+    if (t->join) (void)hipEventDestroy(t->join);
+    if (t->gord) (void)hipEventDestroy(t->gord);
+    delete t;
+}
+
+// Per-disk offsets of nseg disks of counts[s] bodies (NBX_OK, or an
+// argument error with its message).
+int make_segtable(int nseg, const int32_t *counts, SegTable &T)
+{
     if (nseg < 1 || nseg > kMaxSeg) return fail(NBX_E_ARG, "disk count %d not in [1,%d]", nseg,
                                                kMaxSeg);
-    nbx_tree *t = new nbx_tree();
-    SegTable &T = t->T;
+    T = SegTable{};
     T.nseg = nseg;
     long total = 0;
     for (int s = 0; s < nseg; ++s) {
-        if (counts[s] <= 0) {
-            delete t;
-            return fail(NBX_E_ARG, "disk %d has %d bodies", s, counts[s]);
-        }
+        if (counts[s] <= 0) return fail(NBX_E_ARG, "disk %d has %d bodies", s, counts[s]);
         T.off[s] = (int)total;
         total += counts[s];
     }
-    if (total > (1L << 28)) {
-        delete t;
-        return fail(NBX_E_ARG, "too many bodies (%ld)", total);
-    }
+    if (total > (1L << 28)) return fail(NBX_E_ARG, "too many bodies (%ld)", total);
     T.off[nseg] = (int)total;
     T.n_total = (int)total;
-    T.blk[0] = T.wave[0] = T.ioff[0] = T.noff[0] = T.stile[0] = 0;
-    T.bkoff[0] = T.sblk[0] = T.moff[0] = 0;
     for (int s = 0; s < nseg; ++s) {
         const int n = counts[s];
         const int nb = std::min(kMaxBuckets, (n + kBucketMean - 1) / kBucketMean);
@@ -293,33 +314,37 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
         T.sblk[s] = T.sblk[nseg];
         T.moff[s] = T.moff[nseg];
     }
+    return NBX_OK;
+}
+
+int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
+{
+    if (!out) return fail(NBX_E_ARG, "null output pointer");
+    *out = nullptr;
+    SegTable T;
+    int rc0 = make_segtable(nseg, counts, T);
+    if (rc0) return rc0;
+    nbx_tree *t = new nbx_tree();
+    t->T = T;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipStreamCreate(&t->stream);
+    if (e == hipSuccess) e = hipStreamCreate(&t->owned);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->gord, hipEventDisableTiming);
+    t->stream = t->owned;
     if (e != hipSuccess) {
-        delete t;
-        return fail(NBX_E_HIP, "stream setup: %s", hipGetErrorString(e));
-    }
-    t->own_stream = true;
-    if ((e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&t->gord, hipEventDisableTiming)) != hipSuccess) {
         std::string msg = std::string("stream / event setup: ") + hipGetErrorString(e);
-        if (t->side) (void)hipStreamDestroy(t->side);
-        if (t->fork) (void)hipEventDestroy(t->fork);
-        if (t->join) (void)hipEventDestroy(t->join);
-        (void)hipStreamDestroy(t->stream);
-        delete t;
+        destroy_handles(t);
         return fail(NBX_E_HIP, "%s", msg.c_str());
     }
     int rc = alloc_bufs(t);
     if (rc != NBX_OK) {
         std::string msg = g_err;
         free_bufs(t);
-        (void)hipStreamDestroy(t->stream);
-        delete t;
+        destroy_handles(t);
         g_err = msg;
         return rc;
     }
@@ -462,15 +487,9 @@ int nbx_tree_destroy(nbx_tree *t)
 {
     if (!t) return NBX_OK;
     (void)hipStreamSynchronize(t->stream);
-    for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
     (void)hipStreamSynchronize(t->side);
     free_bufs(t);
-    if (t->own_stream) (void)hipStreamDestroy(t->stream);
-    (void)hipStreamDestroy(t->side);
-    (void)hipEventDestroy(t->fork);
-    (void)hipEventDestroy(t->join);
-    (void)hipEventDestroy(t->gord);
-    delete t;
+    destroy_handles(t);
     return NBX_OK;
 }
 
@@ -487,12 +506,9 @@ int nbx_tree_set_stream(nbx_tree *t, void *stream)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
     HIPCHK(hipStreamSynchronize(t->stream));
-    if (stream) {
-        t->stream = (hipStream_t)stream;
-    } else if (!t->own_stream) {
-        HIPCHK(hipStreamCreate(&t->stream));
-        t->own_stream = true;
-    }
+    // the tree's own stream stays alive (and owned) while a caller's stream
+    // is in use; only the owned one is destroyed with the tree
+    t->stream = stream ? (hipStream_t)stream : t->owned;
     return NBX_OK;
 }
 
@@ -1323,3 +1339,4 @@ int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms)
 }
 
 }  // extern "C"
+

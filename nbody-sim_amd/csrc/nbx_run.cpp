@@ -133,7 +133,10 @@ int main(int argc, char **argv)
             if (!output || step0 % every != 0) continue;
             const float sim_time_now = static_cast<float>(step0) * cfg.dt;
             copyToCPU(gpu, cpu);
-            auto aggs = detectAggregates(tree, gpu, cpu, r_coll, cfg.particle_mass_msun);
+            // --plain: the reference's own call of main.cu:148 (host state);
+            // otherwise the partition reuses the step's tree scratch
+            auto aggs = plain ? detectAggregates(cpu, r_coll, cfg)
+                              : detectAggregates(tree, gpu, cpu, r_coll, cfg.particle_mass_msun);
             std::vector<uint16_t> agg_sizes(cfg.n_particles, 1);
             for (const Aggregate &a : aggs) {
                 const uint16_t nc = static_cast<uint16_t>(std::min(a.n_particles, 65535));

@@ -1,11 +1,13 @@
 // nbx_tree.hip -- octree construction and integrator kernels for gfx950.
 //
 // Build pipeline (one launch per phase for all disks of a batch):
-//   per-block boxes -> root cube + 63-bit keys + digit histograms
-//   -> 8 stable radix passes with decoupled look-back (the last one gathers
-//   the sorted bodies) -> FP64 prefix scan of {m, m x, m y, m z} -> binary
-//   radix tree (one thread per internal node) with node mass / COM from the
-//   prefix, octree flags and cell geometry -> record allocation -> records.
+//   kick-drift with per-block boxes -> key pass: root cube, 63-bit keys, each
+//   body scattered to its sort bucket (splitters from the last build's
+//   quantiles) -> bucket sort in LDS (sorted keys, indices and bodies) ->
+//   FP64 prefix scan of {m, m x, m y, m z} -> binary radix tree (one thread
+//   per internal node) with node mass / COM from the prefix, octree flags
+//   and cell geometry -> record allocation -> records.  Disks above 2^20
+//   bodies (or a set digit width) sort with LSD radix passes instead.
 // Every phase is HBM-streaming integer/float work; nothing here is a GEMM.
 #include <float.h>
 
@@ -20,8 +22,11 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ===========================================================================
 // Integrator kernels.  Float expressions are written exactly as in
-// Integrator.cu / Perturber.cu and compiled without contraction so the
-// results round identically to the reference's arithmetic.
+// Integrator.cu / Perturber.cu and compiled without contraction, so every
+// operation rounds as the reference source reads without fused
+// multiply-adds (the oracle evaluates them the same way).  nvcc contracts
+// a * b + c into an FMA by default, so a reference binary may round some of
+// these expressions differently in the last bit.
 // ===========================================================================
 
 __global__ void k_half_kick(float *vx, float *vy, float *vz, const float *__restrict__ ax,

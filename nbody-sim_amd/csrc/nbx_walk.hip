@@ -1,18 +1,16 @@
 // nbx_walk.hip -- tree walks on gfx950: gravity, collision impulse, FoF.
 //
-// Gravity (wave-coherent): a wavefront owns 64 consecutive targets in Morton
-// order (one per lane) and walks the compressed octree with a wave-uniform
-// stack in LDS.  A stack entry is a block of child records {first, count}
-// plus the 64-bit mask of the lanes whose own walk opened the parent.  Each
-// iteration pops up to 64 records from the top of the stack, loads them
-// with one lane-parallel vector load into an LDS staging area, and then
-// evaluates them one after the other with LDS broadcasts; a lane evaluates
-// a record only if its bit is set, so every target visits exactly the nodes
-// of its own per-body walk (BHTree.cu:555-593).
+// Gravity (wave-coherent, nbx_wave.hpp): a wavefront owns TG consecutive
+// targets in Morton order and walks the compressed octree with a
+// wave-uniform stack in LDS; a target evaluates a record only if its own
+// walk reached it, so every target visits exactly the nodes of its own
+// per-body walk (BHTree.cu:555-593).
 //
 // Collision and FoF (per lane): the search radius is far below the particle
-// spacing, so the per-body paths diverge immediately; each lane descends on
-// its own stack, pruning cells by the exact distance to the cell box.
+// spacing, so the per-body paths diverge immediately; each lane climbs to
+// the lowest cell that holds its search sphere and descends below it
+// without a stack (octree parent links), pruning cells by the exact
+// distance to the cell box.
 #include "nbx_wave.hpp"
 
 namespace nbx {
@@ -96,14 +94,17 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     if (n_open) n_open[i] = no;
 }
 
-// Tree arrays of the bottom-up start of the per-lane searches.
+// Tree arrays of the per-lane searches: octree parents (binary index) of
+// octree nodes and of body leaves, every octree node's child block and the
+// codes of its children (kids[8 j + k] is the node or ~body of record
+// first[j] + k), and the cell boxes of the binary nodes.
 struct SearchTree {
-    const int *opar, *opar_leaf, *first, *cnt;
+    const int *opar, *opar_leaf, *first, *cnt, *kids;
     const float4 *bin_cell;
 };
 static inline SearchTree search_bufs(const Bufs &B)
 {
-    return SearchTree{B.opar, B.opar_leaf, B.first, B.cnt, B.bin_cell};
+    return SearchTree{B.opar, B.opar_leaf, B.first, B.cnt, B.kids, B.bin_cell};
 }
 
 // ---------------------------------------------------------------------------
@@ -154,47 +155,69 @@ __device__ inline int search_start(const int *__restrict__ opar, const int *__re
     }
 }
 
-// Depth-first search of the records below one children block {first,
-// count}: the order in which the per-body search from the root meets the
-// pairs inside this block's node.
+// Depth-first search, in pre-order with children in block (Morton) order,
+// of the records below octree node `top` (binary index), without a stack:
+// the lane keeps the node whose child block it scans and the slot in that
+// block; a near internal record is entered at once, and a finished block is
+// left through the node's octree parent (opar) at the slot after the node's
+// own (found among the parent's child codes).  Any depth is handled.  The
+// records of a block are read in chunks of 8, one round trip per chunk.
 template <class Pair>
 __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
-                                       int first0, int count0, float tx, float ty, float tz,
-                                       float radius, float guard, int2 *st, int *err, Pair &pair)
+                                       const SearchTree &S, int top, float tx, float ty, float tz,
+                                       float radius, float guard, Pair &pair)
 {
-    // st: this lane's stack, stride kWave between entries (bank-conflict free)
-    st[0] = make_int2(first0, count0);
-    int sp = 1;
-    while (sp > 0) {
-        --sp;
-        const int2 e = st[sp * kWave];
-        const int first = e.x, count = e.y;
-        for (int k0 = 0; k0 < count; k0 += 8) {
-            int4 B[8];
-            float4 Cc[8];
+    int node = top, f = S.first[top], c = S.cnt[top], k = 0;
+    for (;;) {
+        if (k >= c) {  // block done: back to the parent's next slot
+            if (node == top) return;
+            const int up = S.opar[node];
+            const int4 k0 = *reinterpret_cast<const int4 *>(S.kids + 8 * up);
+            const int4 k1 = *reinterpret_cast<const int4 *>(S.kids + 8 * up + 4);
+            const int fu = S.first[up], cu = S.cnt[up];
+            const int at = k0.x == node   ? 0
+                           : k0.y == node ? 1
+                           : k0.z == node ? 2
+                           : k0.w == node ? 3
+                           : k1.x == node ? 4
+                           : k1.y == node ? 5
+                           : k1.z == node ? 6
+                                          : 7;
+            node = up;
+            f = fu;
+            c = cu;
+            k = at + 1;
+            continue;
+        }
+        int4 B[8];
+        float4 Cc[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                B[k] = rb[first + k0 + k];
-                Cc[k] = rc[first + k0 + k];
-            }
+        for (int q = 0; q < 8; ++q) {
+            B[q] = rb[f + k + q];  // records past the block exist (padding / later blocks)
+            Cc[q] = rc[f + k + q];
+        }
+        const int kend = min(c, k + 8);
+        int down = -1;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k0 + k >= count) break;
-                if (B[k].w == 0) {
-                    pair(Cc[k], B[k].z);
-                } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
-                    if (sp < kLStack) {
-                        st[sp * kWave] = make_int2(B[k].z, B[k].w);
-                        ++sp;
-                    } else {
-                        atomicOr(err, kErrStack);
-                    }
-                }
+        for (int q = 0; q < 8; ++q) {
+            if (k + q >= kend || down >= 0) break;
+            if (B[q].w == 0) {
+                pair(Cc[q], B[q].z);
+            } else if (near_cell(tx, ty, tz, Cc[q], radius, guard)) {
+                down = q;
             }
         }
+        if (down < 0) {
+            k = kend;
+            continue;
+        }
+        // enter the child at slot k + down
+        node = S.kids[8 * node + k + down];
+        f = B[down].z;
+        c = B[down].w;
+        k = 0;
     }
 }
-
 // Collision pass 1 (BHTree.cu:628-672) on the exact pair set
 // dist^2 < r_coll^2: J = -(1+e) dvn / 2 for approaching pairs (dvn < 0),
 // dv_i -= J n, accumulated in the walk's (Morton) order.
@@ -232,11 +255,11 @@ struct CollidePair {
 template <class Pair>
 __device__ inline void lane_search(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
                                    const SearchTree &S, int pos, float tx, float ty, float tz,
-                                   float radius, float guard, int2 *st, int *err, Pair &pair)
+                                   float radius, float guard, Pair &pair)
 {
     const int j = search_start(S.opar, S.opar_leaf, S.bin_cell, pos, tx, ty, tz, radius, guard);
     if (j < 0) return;  // a disk of one body has no pairs
-    lane_walk_block(rc, rb, S.first[j], S.cnt[j], tx, ty, tz, radius, guard, st, err, pair);
+    lane_walk_block(rc, rb, S, j, tx, ty, tz, radius, guard, pair);
 }
 
 __global__ __launch_bounds__(kWalkBlock) void k_collide(
@@ -245,7 +268,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(
     const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
     const Root *__restrict__ roots, float restitution, float *dv, int *err, SearchTree S)
 {
-    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
@@ -258,8 +280,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(
     CollidePair v{b.x, b.y, b.z, r * r, restitution, vx[i], vy[i], vz[i], pos, perm, vx, vy, vz,
                   0.f, 0.f, 0.f, 0};
     if (r > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]),
-                    st + wib * kLStack * kWave + __lane_id(), err, v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]), v);
     const int n = T.n_total;
     dv[i] = v.dvx;
     dv[n + i] = v.dvy;
@@ -328,7 +349,6 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
                                                      const Root *__restrict__ roots, float ll,
                                                      int *parent, int *err, SearchTree S)
 {
-    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
@@ -338,8 +358,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
     const float4 b = bodies[pos];
     FofPair v{b.x, b.y, b.z, ll * ll, perm[pos], perm, parent};
     if (ll > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]),
-                    st + wib * kLStack * kWave + __lane_id(), err, v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]), v);
 }
 
 __global__ void k_iota(int *p, int n)

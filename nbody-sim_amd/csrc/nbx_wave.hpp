@@ -32,7 +32,6 @@ constexpr int kGStack = kStackCap;  // gravity stack entries per wave
 constexpr int kDeepMargin = 7 * 22 + 8;
 constexpr int kBatchMaxSp = kGStack - kDeepMargin - 2 * kWave;
 static_assert(kBatchMaxSp >= 90, "stack too small for batched pops at theta = 0.5");
-constexpr int kLStack = 32;     // per-lane stack entries (collision / FoF)
 
 // A stack entry: child block {first, count} and the mask of the targets that
 // opened its parent.  WIDE (64 targets per wave) keeps mask bits 32..63 in

@@ -553,20 +553,26 @@ def disk_params(r_coll=0.0, perturber=None):
                            perturber.omega)
 
 
-def _disk_array(dps):
+def _disk_array(dps, t):
+    """ctypes array of the disk parameters of tree t: exactly one entry per
+    disk (the library reads one nbx_disk_params per disk of the tree)."""
     if isinstance(dps, nbx_disk_params):
         dps = [dps]
+    dps = list(dps)
+    if len(dps) != len(t.seg_counts):
+        raise ValueError(f"{len(dps)} disk parameter entries for a tree of "
+                         f"{len(t.seg_counts)} disk(s)")
     return (nbx_disk_params * len(dps))(*dps)
 
 
 def init_forces(t, p, sp, dps):
-    _check(lib().nbx_init_forces(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps)),
-           "init_forces")
+    arr = _disk_array(dps, t)
+    _check(lib().nbx_init_forces(t.h, C.byref(p.c()), C.byref(sp), arr), "init_forces")
 
 
 def step(t, p, sp, dps, step0, n_steps):
-    _check(lib().nbx_step(t.h, C.byref(p.c()), C.byref(sp), _disk_array(dps), step0, n_steps),
-           "step")
+    arr = _disk_array(dps, t)
+    _check(lib().nbx_step(t.h, C.byref(p.c()), C.byref(sp), arr, step0, n_steps), "step")
 
 
 def step_concurrent(trees, ps, sp, dps_list, step0, n_steps):
@@ -578,10 +584,10 @@ def step_concurrent(trees, ps, sp, dps_list, step0, n_steps):
     n = len(trees)
     if not (len(ps) == n == len(dps_list)):
         raise ValueError("trees, particles and disk parameters differ in length")
+    arrays = [_disk_array(d, t) for d, t in zip(dps_list, trees)]
     hs = (C.c_void_p * n)(*[t.h for t in trees])
     structs = [p.c() for p in ps]
     pp = (C.POINTER(nbx_particles) * n)(*[C.pointer(c) for c in structs])
-    arrays = [_disk_array(d) for d in dps_list]
     dd = (C.POINTER(nbx_disk_params) * n)(*[C.cast(a, C.POINTER(nbx_disk_params))
                                              for a in arrays])
     _check(lib().nbx_step_concurrent(hs, pp, n, C.byref(sp), dd, step0, n_steps),

@@ -106,7 +106,7 @@ class DiskWindow(torch.autograd.Function):
     def backward(ctx, gx, gv):
         tree, cfg, mass = ctx.tree, ctx.cfg, ctx.mass
         sp, dp = _params(cfg, *ctx.pert)
-        dpa = _disk_array(dp)
+        dpa = _disk_array(dp, tree)
         x_ref = ctx.ckpts[0][0]
         n = x_ref.shape[1]
         ids = torch.arange(n, dtype=torch.int32, device=x_ref.device)

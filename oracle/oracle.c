@@ -852,6 +852,15 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
                  float theta, float softening, int quad, float *self_ax, float *self_ay,
                  float *self_az, int32_t *n_acc, int32_t *n_open, int nthreads)
 {
+    return orc_mt_force_at(x, y, z, m, n, theta, softening, quad, NULL, n, self_ax, self_ay,
+                           self_az, n_acc, n_open, nthreads);
+}
+
+int orc_mt_force_at(const float *x, const float *y, const float *z, const float *m, int n,
+                    float theta, float softening, int quad, const int32_t *targets, int nt,
+                    float *self_ax, float *self_ay, float *self_az, int32_t *n_acc,
+                    int32_t *n_open, int nthreads)
+{
     if (n <= 0) return 0;
     float root[4];
     orc_mt_root(x, y, z, n, root);
@@ -861,6 +870,8 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
     float *sb = (float *)malloc((size_t)n * 4 * sizeof(float));
     orc_mt_keys(x, y, z, n, root, keys);
     orc_mt_sort(keys, n, perm);
+    int32_t *inv = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+    for (int q = 0; q < n; ++q) inv[perm[q]] = q;
     for (int s = 0; s < n; ++s) {
         skeys[s] = keys[perm[s]];
         sb[s] = x[perm[s]];
@@ -882,7 +893,9 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : overflow) num_threads(nthreads)
 #endif
-    for (int s = 0; s < n; ++s) {
+    for (int t = 0; t < nt; ++t) {
+        /* target t: body targets[t] (all bodies in original order without a list) */
+        const int s = inv[targets ? targets[t] : t];
         const float bx = T.sx[s], by = T.sy[s], bz = T.sz[s];
         float fx = 0.f, fy = 0.f, fz = 0.f;
         int acc = 0, opn = 0;
@@ -934,12 +947,11 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
                 }
             }
         }
-        const int i = perm[s];
-        if (self_ax) { self_ax[i] = fx; self_ay[i] = fy; self_az[i] = fz; }
-        if (n_acc) n_acc[i] = acc;
-        if (n_open) n_open[i] = opn;
+        if (self_ax) { self_ax[t] = fx; self_ay[t] = fy; self_az[t] = fz; }
+        if (n_acc) n_acc[t] = acc;
+        if (n_open) n_open[t] = opn;
     }
-    free(keys); free(skeys); free(perm); free(sb); free(T.nodes); free(T.child);
+    free(keys); free(skeys); free(perm); free(inv); free(sb); free(T.nodes); free(T.child);
     return overflow;
 }
 

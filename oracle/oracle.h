@@ -160,6 +160,13 @@ int orc_mt_force(const float *x, const float *y, const float *z, const float *m,
                  float theta, float softening, int quad,
                  float *self_ax, float *self_ay, float *self_az,
                  int32_t *n_acc, int32_t *n_open, int nthreads);
+/* The same walk for the bodies targets[0..nt) only (original indices; the
+ * tree holds all n bodies); outputs are indexed by t < nt.  targets = NULL
+ * with nt = n is orc_mt_force. */
+int orc_mt_force_at(const float *x, const float *y, const float *z, const float *m, int n,
+                    float theta, float softening, int quad, const int32_t *targets, int nt,
+                    float *self_ax, float *self_ay, float *self_az,
+                    int32_t *n_acc, int32_t *n_open, int nthreads);
 
 /* Interaction lists of the same monopole walk: per target (original order)
  * the accepted nodes as sorted-position ranges [lo, hi] (leaf: lo == hi).

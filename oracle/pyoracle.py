@@ -100,6 +100,9 @@ def lib():
         L.orc_mt_force.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_float, C.c_float,
                                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_int]
+        L.orc_mt_force_at.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_float,
+                                      C.c_float, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.orc_mt_lists.restype = C.c_long
         L.orc_mt_lists.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_int, C.c_float, _i32p,
                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]
@@ -272,14 +275,18 @@ def mt_sort(keys):
     return perm
 
 
-def mt_force(x, y, z, m, theta, softening, counts=False, quad=False, nthreads=0):
+def mt_force(x, y, z, m, theta, softening, counts=False, quad=False, nthreads=0, targets=None):
+    """Self-gravity of the compressed-octree walk; with `targets` (original
+    indices) only those bodies are walked and rows follow the target list."""
     x, y, z, m = _f32(x), _f32(y), _f32(z), _f32(m)
     n = len(x)
-    sx, sy, sz = (np.zeros(n, np.float32) for _ in range(3))
-    na = np.zeros(n, np.int32) if counts else None
-    no = np.zeros(n, np.int32) if counts else None
-    ov = lib().orc_mt_force(x, y, z, m, n, theta, softening, int(quad), _ptr(sx), _ptr(sy), _ptr(sz),
-                            _ptr(na), _ptr(no), nthreads)
+    tg = None if targets is None else np.ascontiguousarray(targets, np.int32)
+    nt = n if tg is None else len(tg)
+    sx, sy, sz = (np.zeros(nt, np.float32) for _ in range(3))
+    na = np.zeros(nt, np.int32) if counts else None
+    no = np.zeros(nt, np.int32) if counts else None
+    ov = lib().orc_mt_force_at(x, y, z, m, n, theta, softening, int(quad), _ptr(tg), nt,
+                               _ptr(sx), _ptr(sy), _ptr(sz), _ptr(na), _ptr(no), nthreads)
     out = dict(self_a=np.stack([sx, sy, sz], 1), overflow=ov)
     if counts:
         out["n_acc"], out["n_open"] = na, no

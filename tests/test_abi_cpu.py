@@ -123,3 +123,26 @@ def test_step_params_layout_matches_header():
         assert body, struct
         fields = re.findall(r"\b([a-z_0-9]+)\s*[,;]", re.sub(r"/\*.*?\*/", "", body.group(1)))
         assert fields == [f[0] for f in cls._fields_], (struct, fields)
+
+
+def test_disk_params_length_checked_before_device_work():
+    """The library reads one nbx_disk_params per disk of a tree: the Python
+    wrappers refuse a list of another length before any call (a shorter
+    array would be read past its end)."""
+    class FakeTree:
+        seg_counts = [100, 200, 300]
+        h = None
+    dp = nbx.disk_params(1e-3)
+    for fn in (lambda t, d: nbx.init_forces(t, None, None, d),
+               lambda t, d: nbx.step(t, None, None, d, 0, 1)):
+        with pytest.raises(ValueError, match="3 disk"):
+            fn(FakeTree(), dp)
+        with pytest.raises(ValueError, match="2 disk parameter entries"):
+            fn(FakeTree(), [dp, dp])
+    with pytest.raises(ValueError, match="1 disk parameter entries"):
+        nbx.step_concurrent([FakeTree()], [None], None, [[dp]], 0, 1)
+    one = FakeTree()
+    one.seg_counts = [10]
+    assert len(nbx._disk_array(dp, one)) == 1
+    assert len(nbx._disk_array([dp, dp, dp], FakeTree())) == 3
+

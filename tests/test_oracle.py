@@ -364,3 +364,16 @@ def test_reference_loop_two_body_kepler():
     r = math.sqrt(float(s["x"][0]) ** 2 + float(s["y"][0]) ** 2 + float(s["z"][0]) ** 2)
     e = 0.5 * (float(s["vx"][0]) ** 2 + float(s["vy"][0]) ** 2 + float(s["vz"][0]) ** 2) - 1 / r
     assert abs(e + 0.5) < 1e-3
+
+
+def test_mt_force_target_subset_equals_full_rows():
+    """The walk over a target list gives exactly the rows of the full walk
+    (same tree, same per-body walk), counts included."""
+    from helpers import disk
+    d = disk(6000, seed=17)
+    full = O.mt_force(d["x"], d["y"], d["z"], d["mass"], 0.5, 1e-3, counts=True)
+    tg = np.random.default_rng(3).choice(6000, 700, replace=False)
+    sub = O.mt_force(d["x"], d["y"], d["z"], d["mass"], 0.5, 1e-3, counts=True, targets=tg)
+    np.testing.assert_array_equal(sub["self_a"], full["self_a"][tg])
+    np.testing.assert_array_equal(sub["n_acc"], full["n_acc"][tg])
+    np.testing.assert_array_equal(sub["n_open"], full["n_open"][tg])
