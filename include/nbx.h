@@ -148,6 +148,20 @@ int nbx_gas_drag(nbx_particles *p, float drag_rate, float dt,
  * groups label themselves).  Builds its own tree from p. */
 int nbx_fof(nbx_tree *t, const nbx_particles *p, float link_length, int32_t *dev_labels);
 
+/* No counterpart in the reference: nbx_fof for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same labels as nbx_fof, with no tree: the scratch is a
+ * static device array of 20 MiB per GPU (allocated when the library is
+ * loaded, held until the process ends), so at most NBX_NOALLOC_MAX_BODIES
+ * bodies (larger n: NBX_E_ARG before any device work).  No hipMalloc,
+ * hipFree or other allocating call.  Work goes on `stream` (a hipStream_t;
+ * NULL = the null stream) and the call returns after it completed.  Every
+ * call on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (other threads, other streams) race and must be serialized by the
+ * caller. */
+#define NBX_NOALLOC_MAX_BODIES 65536
+int nbx_fof_noalloc(const nbx_particles *p, float link_length, int32_t *dev_labels, void *stream);
+
 /* ---- fused step (main.cu:101-140) ---------------------------------------- */
 typedef struct nbx_step_params {
     float dt, theta, softening, star_mass;

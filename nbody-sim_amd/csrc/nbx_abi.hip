@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1340,3 +1341,108 @@ int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms)
 
 }  // extern "C"
 
+// ===========================================================================
+// FoF labels without allocation (static device scratch)
+// ===========================================================================
+namespace {
+
+// Bytes of the static scratch: the core layout of one disk of
+// NBX_NOALLOC_MAX_BODIES bodies (about 284 bytes per body plus the fixed
+// histogram and padding) fits with margin; checked when a device first
+// uses it.
+constexpr size_t kNoAllocScratchBytes = (size_t)20 << 20;
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+// One copy per GPU, alive until the process ends.  Host code never uses the
+// name itself (a host-side stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Bump layout of the core buffers over `base` (nullptr: sizes only, every
+// pointer stays null); returns the bytes used.  Each buffer starts on a
+// 256-byte boundary.
+size_t carve_core(const SegTable &T, Bufs &B, unsigned char *base)
+{
+    size_t off = 0;
+    auto take = [&](auto *&ptr, size_t count) {
+        using Elem = std::remove_reference_t<decltype(*ptr)>;
+        off = (off + 255) & ~(size_t)255;
+        ptr = base ? reinterpret_cast<Elem *>(base + off) : nullptr;
+        off += (count ? count : 1) * sizeof(Elem);
+    };
+    core_layout(T, B, take);
+    if (base) scan_views(B);
+    return off;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_of_current_device(unsigned char **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(NBX_E_ARG, "device %d not supported", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(NBX_E_HIP, "static scratch has no device address");
+        // the largest disk the entry point accepts must fit
+        SegTable T;
+        const int32_t nmax = NBX_NOALLOC_MAX_BODIES;
+        int rc = make_segtable(1, &nmax, T);
+        if (rc) return rc;
+        Bufs B;
+        const size_t need = carve_core(T, B, nullptr);
+        if (need > kNoAllocScratchBytes)
+            return fail(NBX_E_STATE, "static scratch of %zu bytes holds %zu needed (internal)",
+                        kNoAllocScratchBytes, need);
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[dev]);
+    return NBX_OK;
+}
+
+}  // namespace
+
+extern "C" int nbx_fof_noalloc(const nbx_particles *p, float link_length, int32_t *dev_labels,
+                               void *stream)
+{
+    if (!particles_ok(p)) return fail(NBX_E_ARG, "nbx_fof_noalloc: invalid particle arrays");
+    if (!dev_labels) return fail(NBX_E_ARG, "nbx_fof_noalloc: null labels");
+    if (p->n > NBX_NOALLOC_MAX_BODIES)
+        return fail(NBX_E_ARG,
+                    "nbx_fof_noalloc: %d bodies exceed the static scratch's maximum of %d "
+                    "(use nbx_fof with a tree)",
+                    p->n, NBX_NOALLOC_MAX_BODIES);
+    SegTable T;
+    int rc = make_segtable(1, &p->n, T);
+    if (rc) return rc;
+    unsigned char *base = nullptr;
+    if ((rc = scratch_of_current_device(&base)) != NBX_OK) return rc;
+    Bufs B;
+    if (carve_core(T, B, base) > kNoAllocScratchBytes)
+        return fail(NBX_E_STATE, "nbx_fof_noalloc: scratch layout overflow (internal)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // the previous call's contents stay in the scratch: the error flags
+    // start from zero (the key pass resets its own tickets and allocators)
+    launch_fill_int(B.err, 0, 4, st);
+    launch_bbox(T, p->x, p->y, p->z, B, st);
+    launch_root(T, B, st);
+    launch_build(T, p->x, p->y, p->z, p->mass, B, st);
+    launch_summarize(T, B, st);
+    launch_fof(T, link_length, dev_labels, B, st);
+    HIPCHK(hipGetLastError());
+    int h_err[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h_err, B.err, sizeof(h_err), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (h_err[0] & kErrTree) return fail(NBX_E_STATE, "inconsistent octree (internal error)");
+    return NBX_OK;
+}

@@ -112,7 +112,7 @@ EXPORTS = [
     "nbx_adj_init_forces", "nbx_walk_wave_stats", "nbx_config_default", "nbx_config_load",
     "nbx_particle_radius", "nbx_init_disk", "nbx_writer_open", "nbx_writer_frame",
     "nbx_writer_stats", "nbx_writer_perturber", "nbx_writer_finalize", "nbx_writer_close",
-    "nbx_write_perturber_sidecar",
+    "nbx_write_perturber_sidecar", "nbx_fof_noalloc",
 ]
 
 _lib = None
@@ -142,6 +142,7 @@ def lib():
             "nbx_collide": [vp, P, f32, f32], "nbx_perturber_force": [P, f32, f32, f32, f32, f32],
             "nbx_half_kick": [P, f32], "nbx_drift": [P, f32], "nbx_reset_acceleration": [P],
             "nbx_gas_drag": [P, f32, f32, f32], "nbx_fof": [vp, P, f32, vp],
+            "nbx_fof_noalloc": [P, f32, vp, vp],
             "nbx_init_forces": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params)],
             "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
             "nbx_step_concurrent": [C.POINTER(vp), C.POINTER(P), i32, C.POINTER(nbx_step_params),
@@ -490,6 +491,19 @@ def walk_wave_stats(t, theta):
 def fof_into(t, p, link_length, labels):
     """FoF labels into an existing int32 device tensor (no allocation)."""
     _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, labels.data_ptr()), "fof")
+
+
+NOALLOC_MAX_BODIES = 65536  # NBX_NOALLOC_MAX_BODIES of include/nbx.h
+
+
+def fof_noalloc(p, link_length, labels, stream=None):
+    """FoF labels into an existing int32 device tensor without a tree and
+    without any device allocation (static scratch, at most
+    NOALLOC_MAX_BODIES bodies); `stream` a torch stream or None (the null
+    stream).  Calls that overlap on one GPU race."""
+    s = None if stream is None else C.c_void_p(stream.cuda_stream)
+    _check(lib().nbx_fof_noalloc(C.byref(p.c()), link_length, labels.data_ptr(), s),
+           "fof_noalloc")
 
 
 def fof_labels(t, p, link_length):

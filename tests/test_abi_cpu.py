@@ -146,3 +146,22 @@ def test_disk_params_length_checked_before_device_work():
     assert len(nbx._disk_array(dp, one)) == 1
     assert len(nbx._disk_array([dp, dp, dp], FakeTree())) == 3
 
+
+def test_fof_noalloc_rejects_oversized_input_before_device_work():
+    """The static-scratch FoF takes at most NBX_NOALLOC_MAX_BODIES bodies:
+    a larger count is an argument error, reported without touching a
+    device (the pointers are never dereferenced)."""
+    L = nbx.lib()
+    text = open(os.path.join(REPO, "include", "nbx.h")).read()
+    assert f"#define NBX_NOALLOC_MAX_BODIES {nbx.NOALLOC_MAX_BODIES}" in text
+    s = nbx.nbx_particles()
+    for k in nbx.FIELDS:
+        setattr(s, k, 16)  # non-null stand-ins, never read
+    s.id = 16
+    s.n = nbx.NOALLOC_MAX_BODIES + 1
+    assert L.nbx_fof_noalloc(C.byref(s), 1e-3, C.c_void_p(16), None) == -1
+    assert b"exceed" in L.nbx_last_error()
+    assert L.nbx_fof_noalloc(None, 1e-3, C.c_void_p(16), None) == -1
+    s.n = 100
+    assert L.nbx_fof_noalloc(C.byref(s), 1e-3, None, None) == -1
+    assert b"null labels" in L.nbx_last_error()
