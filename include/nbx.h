@@ -127,6 +127,11 @@ int nbx_walk_stats(nbx_tree *t, float theta, double *out);
  * *n_waves receives the wavefront count; out may be NULL to query it, else
  * it must hold cap >= *n_waves wavefronts. */
 int nbx_walk_wave_stats(nbx_tree *t, float theta, int32_t *out, int32_t cap, int32_t *n_waves);
+/* No reference counterpart (diagnostic): out[0] = colliding pairs counted by
+ * the collision passes, out[1] = per-lane searches (collision / FoF) that
+ * went on below a record without the per-lane stack (a full stack), both
+ * since the tree's creation or the last clear; clear=1 zeroes them. */
+int nbx_search_stats(nbx_tree *t, int32_t *out, int clear);
 /* launchCollisionKernel (BHTree.cu:688-708): exact pair set
  * dist^2 < r_coll^2, impulse J = -(1+e) dvn / 2 for approaching pairs,
  * deltas accumulated per body in a fixed order, then v += dv. */
@@ -147,6 +152,11 @@ int nbx_gas_drag(nbx_particles *p, float drag_rate, float dt,
  * the device: dev_labels[i] = smallest particle index in i's group (1-body
  * groups label themselves).  Builds its own tree from p. */
 int nbx_fof(nbx_tree *t, const nbx_particles *p, float link_length, int32_t *dev_labels);
+/* No reference counterpart (batching): nbx_fof with link_lengths[s] for
+ * disk s of a batched tree (one entry per disk; the reference links at each
+ * run's own r_coll, main.cu:148).  Disks never link to each other. */
+int nbx_fof_disks(nbx_tree *t, const nbx_particles *p, const float *link_lengths,
+                  int32_t *dev_labels);
 
 /* No counterpart in the reference: nbx_fof for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that

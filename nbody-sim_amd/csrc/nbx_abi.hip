@@ -822,6 +822,22 @@ int nbx_walk_wave_stats(nbx_tree *t, float theta, int32_t *out, int32_t cap, int
     return finish(t);
 }
 
+int nbx_search_stats(nbx_tree *t, int32_t *out, int clear)
+{
+    if (!t || !out) return fail(NBX_E_ARG, "null argument");
+    int h[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, t->B.err, sizeof(h), hipMemcpyDeviceToHost, t->stream));
+    HIPCHK(hipStreamSynchronize(t->stream));
+    out[0] = h[1];
+    out[1] = h[3];
+    if (clear) {
+        launch_fill_int(t->B.err + 1, 0, 1, t->stream);
+        launch_fill_int(t->B.err + 3, 0, 1, t->stream);
+        HIPCHK(hipStreamSynchronize(t->stream));
+    }
+    return NBX_OK;
+}
+
 int nbx_collide(nbx_tree *t, nbx_particles *p, float r_coll, float restitution)
 {
     int rc = check_tree_particles(t, p);
@@ -878,15 +894,25 @@ int nbx_gas_drag(nbx_particles *p, float drag_rate, float dt, float star_mass)
 
 int nbx_fof(nbx_tree *t, const nbx_particles *p, float link_length, int32_t *dev_labels)
 {
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    std::vector<float> ll(t->T.nseg, link_length);
+    return nbx_fof_disks(t, p, ll.data(), dev_labels);
+}
+
+int nbx_fof_disks(nbx_tree *t, const nbx_particles *p, const float *link_lengths,
+                  int32_t *dev_labels)
+{
     int rc = check_tree_particles(t, p);
     if (rc) return rc;
-    if (!dev_labels) return fail(NBX_E_ARG, "null labels");
+    if (!dev_labels || !link_lengths) return fail(NBX_E_ARG, "null labels or link lengths");
+    LinkTable LT{};
+    for (int s = 0; s < t->T.nseg; ++s) LT.ll[s] = link_lengths[s];
     launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
     launch_root(t->T, t->B, t->stream);
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
-    launch_fof(t->T, link_length, dev_labels, t->B, t->stream);
+    launch_fof(t->T, LT, dev_labels, t->B, t->stream);
     return finish(t);
 }
 
@@ -1438,7 +1464,9 @@ extern "C" int nbx_fof_noalloc(const nbx_particles *p, float link_length, int32_
     launch_root(T, B, st);
     launch_build(T, p->x, p->y, p->z, p->mass, B, st);
     launch_summarize(T, B, st);
-    launch_fof(T, link_length, dev_labels, B, st);
+    LinkTable LT{};
+    LT.ll[0] = link_length;
+    launch_fof(T, LT, dev_labels, B, st);
     HIPCHK(hipGetLastError());
     int h_err[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(h_err, B.err, sizeof(h_err), hipMemcpyDeviceToHost, st));

@@ -8,7 +8,8 @@
 //   bodies     float4 {x,y,z,m} gathered in sorted order (walk leaves)
 //   bin_*      binary radix tree over the sorted keys (n-1 nodes per disk)
 //   rec_a/b    walk records of the compressed octree: float4 {com, mass}
-//              and int4 {h, h^2, first child record, child count}; the
+//              and int4 {code, h^2, first child record, child count} (code:
+//              the binary node of the record, ~sorted position for a leaf); the
 //              children of a node are contiguous, the root record of disk s
 //              sits at node_off[s].
 #pragma once
@@ -118,6 +119,11 @@ struct DiskDev {
 
 struct DiskTable {
     DiskDev d[kMaxSeg];
+};
+
+// FoF link length per disk (passed by value).
+struct LinkTable {
+    float ll[kMaxSeg];
 };
 
 // Root cube per disk: {cx, cy, cz, half}.

@@ -55,7 +55,7 @@ struct Bufs {
     float4 *bin_com = nullptr;   // [n] {com, mass} of binary nodes
     float4 *bin_cell = nullptr;  // [n] {cell centre, half size} of binary nodes
     float4 *rec_a = nullptr;     // walk records {com, mass}
-    int4 *rec_b = nullptr;       // walk records {h, h^2, first, count}
+    int4 *rec_b = nullptr;       // walk records {code, h^2, first, count}
     float4 *rec_c = nullptr;     // walk records {cell centre | body, h}
     int n_rec = 0;
     float *partial = nullptr;    // [blocks * 6] per-block bbox min / max
@@ -179,8 +179,8 @@ double probe_fp32(int kind, float *scratch, hipStream_t st);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,
                     const float *vz, float restitution, Bufs &B, hipStream_t st);
-// FoF labels (original order) with link length ll for every disk.
-void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t st);
+// FoF labels (original order), disk s linked at LT.ll[s].
+void launch_fof(const SegTable &T, const LinkTable &LT, int *labels, Bufs &B, hipStream_t st);
 
 // ---- reverse mode (collision-free step, monopole walk) ---------------------
 struct AdjWalkArgs {

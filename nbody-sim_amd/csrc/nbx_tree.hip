@@ -1659,16 +1659,18 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
             rec_q[2 * r + 1] = bin_q[2 * code + 1];
         }
     };
+    // rec_b = {code, h^2, first, count}: the record's own code (binary node,
+    // or ~sorted position of a leaf) lets a search continue below any record
     auto rec_of = [&](int code, float4 &a, int4 &b, float4 &c) {
         if (code < 0) {
             const int p = ~code;
             a = bodies[p];
-            b = make_int4(__float_as_int(0.f), __float_as_int(-1.f), p, 0);
+            b = make_int4(code, __float_as_int(-1.f), p, 0);
             c = make_float4(a.x, a.y, a.z, 0.f);
         } else {
             c = bin_cell[code];
             a = bin_com[code];
-            b = make_int4(__float_as_int(c.w), __float_as_int(c.w * c.w), first[code], cnt[code]);
+            b = make_int4(code, __float_as_int(c.w * c.w), first[code], cnt[code]);
         }
     };
     if (k == 0 && j < T.nseg && T.off[j + 1] - T.off[j] == 1) {

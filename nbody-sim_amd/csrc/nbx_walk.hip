@@ -94,6 +94,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
     if (n_open) n_open[i] = no;
 }
 
+constexpr int kLStack = 32;  // per-lane stack entries of the searches (LDS)
+
 // Tree arrays of the per-lane searches: octree parents (binary index) of
 // octree nodes and of body leaves, every octree node's child block and the
 // codes of its children (kids[8 j + k] is the node or ~body of record
@@ -163,9 +165,10 @@ __device__ inline int search_start(const int *__restrict__ opar, const int *__re
 // own (found among the parent's child codes).  Any depth is handled.  The
 // records of a block are read in chunks of 8, one round trip per chunk.
 template <class Pair>
-__device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
-                                       const SearchTree &S, int top, float tx, float ty, float tz,
-                                       float radius, float guard, Pair &pair)
+__device__ inline void lane_walk_subtree(const float4 *__restrict__ rc,
+                                         const int4 *__restrict__ rb, const SearchTree &S,
+                                         int top, float tx, float ty, float tz, float radius,
+                                         float guard, Pair &pair)
 {
     int node = top, f = S.first[top], c = S.cnt[top], k = 0;
     for (;;) {
@@ -211,13 +214,59 @@ __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4
             k = kend;
             continue;
         }
-        // enter the child at slot k + down
-        node = S.kids[8 * node + k + down];
+        node = B[down].x;  // the record's own binary node
         f = B[down].z;
         c = B[down].w;
         k = 0;
     }
 }
+
+// Depth-first search of the records below one children block {first,
+// count} on a per-lane stack of kLStack blocks in LDS: a block is scanned
+// whole (one round trip per 8 records) and the near internal records among
+// its records are pushed.  A lane whose stack is full (a clump with every
+// cell in range: up to 7 pending blocks per level) searches that record's
+// subtree at once without a stack, so no depth or clump is out of reach.
+template <class Pair>
+__device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
+                                       const SearchTree &S, int first0, int count0, float tx,
+                                       float ty, float tz, float radius, float guard, int2 *st,
+                                       int *err, Pair &pair)
+{
+    // st: this lane's stack, stride kWave between entries (bank-conflict free)
+    st[0] = make_int2(first0, count0);
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        const int2 e = st[sp * kWave];
+        const int first = e.x, count = e.y;
+        for (int k0 = 0; k0 < count; k0 += 8) {
+            int4 B[8];
+            float4 Cc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                B[k] = rb[first + k0 + k];
+                Cc[k] = rc[first + k0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k0 + k >= count) break;
+                if (B[k].w == 0) {
+                    pair(Cc[k], B[k].z);
+                } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
+                    if (sp < kLStack) {
+                        st[sp * kWave] = make_int2(B[k].z, B[k].w);
+                        ++sp;
+                    } else {
+                        atomicAdd(err + 3, 1);  // diagnostic: searches past the stack
+                        lane_walk_subtree(rc, rb, S, B[k].x, tx, ty, tz, radius, guard, pair);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Collision pass 1 (BHTree.cu:628-672) on the exact pair set
 // dist^2 < r_coll^2: J = -(1+e) dvn / 2 for approaching pairs (dvn < 0),
 // dv_i -= J n, accumulated in the walk's (Morton) order.
@@ -255,11 +304,11 @@ struct CollidePair {
 template <class Pair>
 __device__ inline void lane_search(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
                                    const SearchTree &S, int pos, float tx, float ty, float tz,
-                                   float radius, float guard, Pair &pair)
+                                   float radius, float guard, int2 *st, int *err, Pair &pair)
 {
     const int j = search_start(S.opar, S.opar_leaf, S.bin_cell, pos, tx, ty, tz, radius, guard);
     if (j < 0) return;  // a disk of one body has no pairs
-    lane_walk_block(rc, rb, S, j, tx, ty, tz, radius, guard, pair);
+    lane_walk_block(rc, rb, S, S.first[j], S.cnt[j], tx, ty, tz, radius, guard, st, err, pair);
 }
 
 __global__ __launch_bounds__(kWalkBlock) void k_collide(
@@ -268,6 +317,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(
     const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
     const Root *__restrict__ roots, float restitution, float *dv, int *err, SearchTree S)
 {
+    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
@@ -280,7 +330,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(
     CollidePair v{b.x, b.y, b.z, r * r, restitution, vx[i], vy[i], vz[i], pos, perm, vx, vy, vz,
                   0.f, 0.f, 0.f, 0};
     if (r > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]), v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]),
+                    st + wib * kLStack * kWave + __lane_id(), err, v);
     const int n = T.n_total;
     dv[i] = v.dvx;
     dv[n + i] = v.dvy;
@@ -346,9 +397,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
                                                      const int4 *__restrict__ rb,
                                                      const float4 *__restrict__ bodies,
                                                      const int *__restrict__ perm,
-                                                     const Root *__restrict__ roots, float ll,
+                                                     const Root *__restrict__ roots, LinkTable LT,
                                                      int *parent, int *err, SearchTree S)
 {
+    __shared__ int2 st[kLStack * kWalkBlock];
     const int wib = threadIdx.x >> 6;
     const int w = blockIdx.x * kWalkWaves + wib;
     if (w >= T.wave[T.nseg]) return;
@@ -356,9 +408,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
     const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
     if (pos >= T.off[s + 1]) return;
     const float4 b = bodies[pos];
+    const float ll = LT.ll[s];  // the disk's own link length
     FofPair v{b.x, b.y, b.z, ll * ll, perm[pos], perm, parent};
     if (ll > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]), v);
+        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]),
+                    st + wib * kLStack * kWave + __lane_id(), err, v);
 }
 
 __global__ void k_iota(int *p, int n)
@@ -570,12 +624,12 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
                        B.dv, B.err, search_bufs(B));
 }
 
-void launch_fof(const SegTable &T, float ll, int *labels, Bufs &B, hipStream_t st)
+void launch_fof(const SegTable &T, const LinkTable &LT, int *labels, Bufs &B, hipStream_t st)
 {
     const int n = T.n_total, waves = T.wave[T.nseg];
     hipLaunchKernelGGL(k_iota, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels, n);
     hipLaunchKernelGGL(k_fof_link, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
-                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], B.roots, ll, B.labels, B.err,
+                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], B.roots, LT, B.labels, B.err,
                        search_bufs(B));
     hipLaunchKernelGGL(k_fof_final, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, B.labels,
                        labels, n);

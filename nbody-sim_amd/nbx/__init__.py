@@ -112,7 +112,7 @@ EXPORTS = [
     "nbx_adj_init_forces", "nbx_walk_wave_stats", "nbx_config_default", "nbx_config_load",
     "nbx_particle_radius", "nbx_init_disk", "nbx_writer_open", "nbx_writer_frame",
     "nbx_writer_stats", "nbx_writer_perturber", "nbx_writer_finalize", "nbx_writer_close",
-    "nbx_write_perturber_sidecar", "nbx_fof_noalloc",
+    "nbx_write_perturber_sidecar", "nbx_fof_noalloc", "nbx_fof_disks", "nbx_search_stats",
 ]
 
 _lib = None
@@ -143,6 +143,8 @@ def lib():
             "nbx_half_kick": [P, f32], "nbx_drift": [P, f32], "nbx_reset_acceleration": [P],
             "nbx_gas_drag": [P, f32, f32, f32], "nbx_fof": [vp, P, f32, vp],
             "nbx_fof_noalloc": [P, f32, vp, vp],
+            "nbx_fof_disks": [vp, P, C.POINTER(f32), vp],
+            "nbx_search_stats": [vp, C.POINTER(i32), i32],
             "nbx_init_forces": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params)],
             "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
             "nbx_step_concurrent": [C.POINTER(vp), C.POINTER(P), i32, C.POINTER(nbx_step_params),
@@ -470,6 +472,14 @@ def walk_counts(t, p, theta):
     return float(a.double().mean()), float(o.double().mean())
 
 
+def search_stats(t, clear=True):
+    """{pairs: colliding pairs counted, unstacked: per-lane searches that
+    went on below a record without the per-lane stack} since the last clear."""
+    out = (C.c_int32 * 2)()
+    _check(lib().nbx_search_stats(t.h, out, int(clear)), "search_stats")
+    return dict(pairs=out[0], unstacked=out[1])
+
+
 def walk_stats(t, theta):
     """{acc, open: mean per body; records_per_wave; max_stack} of the theta walk."""
     out = (C.c_double * 4)()
@@ -489,8 +499,17 @@ def walk_wave_stats(t, theta):
 
 
 def fof_into(t, p, link_length, labels):
-    """FoF labels into an existing int32 device tensor (no allocation)."""
-    _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, labels.data_ptr()), "fof")
+    """FoF labels into an existing int32 device tensor (no allocation).
+    link_length: one value for every disk, or a list with one per disk of
+    the tree."""
+    if np.ndim(link_length) == 0:
+        _check(lib().nbx_fof(t.h, C.byref(p.c()), link_length, labels.data_ptr()), "fof")
+        return
+    ll = [float(v) for v in link_length]
+    if len(ll) != len(t.seg_counts):
+        raise ValueError(f"{len(ll)} link lengths for a tree of {len(t.seg_counts)} disk(s)")
+    arr = (C.c_float * len(ll))(*ll)
+    _check(lib().nbx_fof_disks(t.h, C.byref(p.c()), arr, labels.data_ptr()), "fof")
 
 
 NOALLOC_MAX_BODIES = 65536  # NBX_NOALLOC_MAX_BODIES of include/nbx.h

@@ -182,8 +182,9 @@ def test_sweep_members_equal_plain_runs(gpu):
     drag) stepped 20 times three ways: each alone (its own tree), all six
     in one batched tree, and as two batched groups stepped side by side on
     their own streams (the sweep's grouping).  Every member equals its plain
-    run bit for bit in all nine state arrays, and its FoF labels at 0.0261 AU
-    on the batched tree equal the oracle's partition of its positions."""
+    run bit for bit in all nine state arrays, and its FoF labels on the
+    batched tree (each disk at its own link length, 0.005-0.0261 AU) equal
+    the oracle's partition of its positions."""
     nbx = gpu
     disks = [disk(n, mass=3e-5, seed=100 + i) for i, n in enumerate(SWEEP_N)]
     sp = _gpu_params(nbx, use_graph=True)
@@ -218,12 +219,14 @@ def test_sweep_members_equal_plain_runs(gpu):
     nbx.step(tb, pb, sp, dps, 0, 20)
     gb = pb.numpy()
     check(gb, "batched tree")
-    _say("FoF of the batched tree at 0.0261 AU")
-    lab = _np(nbx.fof_labels(tb, pb, LL_PLANET))
+    _say("FoF of the batched tree, one link length per disk")
+    lls = [LL_PLANET, 0.01, LL_PLANET, 0.005, 0.02, LL_PLANET]
+    lab_t = torch.empty(pb.n, dtype=torch.int32, device="cuda")
+    nbx.fof_into(tb, pb, lls, lab_t)
+    lab = _np(lab_t)
     off = 0
-    for i, n in enumerate(SWEEP_N):
-        ref, _ = O.fof(gb["x"][off:off + n], gb["y"][off:off + n], gb["z"][off:off + n],
-                       LL_PLANET)
+    for i, (n, ll) in enumerate(zip(SWEEP_N, lls)):
+        ref, _ = O.fof(gb["x"][off:off + n], gb["y"][off:off + n], gb["z"][off:off + n], ll)
         np.testing.assert_array_equal(lab[off:off + n] - off, ref, err_msg=f"FoF disk {i}")
         off += n
     tb.free()
@@ -301,8 +304,10 @@ def _deep_clump(n_bg, n_clump, seed):
 @pytest.mark.parametrize("ll", [1e-2, 1e-4, 1e-6])
 def test_deep_clump_fof_and_collisions(gpu, ll):
     """FoF and collision searches below a clump of 3,000 bodies spread over
-    five decades of scale (every cell of the clump within the search radius
-    at ll = 1e-2, partial at 1e-4 and 1e-6): labels bit-exact against the
+    three decades of scale (every cell of the clump within the search radius
+    at ll = 1e-2, where the pending blocks overflow the per-lane stack and
+    the search goes on without it; partial at 1e-4 and 1e-6): the pair count
+    equal to the oracle's, labels bit-exact against the
     oracle, collision deltas against the oracle's exact pair set within
     1e-5 max |v| (a clump body sums up to 3,000 impulses, in Morton
     pre-order here and in the reference's stack order there; measured
@@ -313,7 +318,11 @@ def test_deep_clump_fof_and_collisions(gpu, ll):
     p = nbx.to_gpu(d)
     t = nbx.allocateOctree(n)
     _say(f"deep clump ll={ll}")
+    nbx.search_stats(t, clear=True)
     lab = _np(nbx.fof_labels(t, p, ll))
+    unstacked = nbx.search_stats(t)["unstacked"]
+    if ll == 1e-2:  # pending blocks pile up past the per-lane stack
+        assert unstacked > 0
     ref, _ = O.fof(d["x"], d["y"], d["z"], ll, method="grid")
     np.testing.assert_array_equal(lab, ref)
     if ll == 1e-2:
@@ -321,11 +330,14 @@ def test_deep_clump_fof_and_collisions(gpu, ll):
     nbx.launchBBoxKernel(t, p)
     nbx.launchBuildKernel(t, p)
     nbx.launchSummarizeKernel(t)
+    nbx.search_stats(t, clear=True)
     nbx.launchCollisionKernel(t, p, ll, 0.2)
     g = p.numpy()
     rt = O.RefTree(d["x"], d["y"], d["z"], d["mass"])
     assert rt.lost == 0  # the oracle's canonical octree holds every body
     vx, vy, vz, pairs = rt.collide(d["vx"], d["vy"], d["vz"], ll, 0.2, exact=True)
+    # every ordered pair counted once by each of its bodies
+    assert nbx.search_stats(t)["pairs"] == pairs
     gv = np.stack([g["vx"], g["vy"], g["vz"]], 1)
     ov = np.stack([vx, vy, vz], 1)
     assert np.abs(gv - ov).max() <= 1e-5 * np.abs(ov).max()

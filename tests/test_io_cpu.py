@@ -222,3 +222,49 @@ def test_perturber_sidecar(tmp_path):
     j = json.loads((tmp_path / "perturber.json").read_text())
     assert j["enabled"] is True and abs(j["mass_msun"] - 9.5479e-3) < 1e-9
     assert set(j) == {"enabled", "mass_msun", "r_AU", "phase0_rad", "omega_T0_inv"}
+
+
+def test_stats_reader_dtypes_chunks_and_summary(tmp_path):
+    """The streaming stats.csv reader returns the analysis scripts' columns
+    and dtypes (analyze_experiment.py:29-36) and regroups frames across
+    chunk boundaries: every chunk size gives the same per-frame table, whose
+    values follow the rows written (aggregate count, three largest by
+    particle count, perturber row apart)."""
+    from nbx import stats
+    rng = np.random.default_rng(4)
+    n, dt, every = 50, f32(0.01), 10
+    w = nbx.OutputWriter(str(tmp_path), n, dt, every)
+    expect = []
+    for fr in range(6):
+        k = int(rng.integers(0, 9))  # frames with 0 .. 8 aggregates
+        sizes = rng.integers(2, 500, k)
+        aggs = [nbx.Aggregate(j, int(s), 1e-6 * int(s), 0.1, 0.0, 0.2, 0.0, 0.0, 0.0, 0.2, [])
+                for j, s in enumerate(sizes)]
+        t = float(f32(0.1) * (fr + 1))
+        w.writeStats(aggs, fr, t)
+        if fr != 3:  # one frame without the perturber row
+            w.writePerturberStats(fr, t, len(aggs), 4.0, 0.0, 1.0, 9.5479e-3, 0.08)
+        top = sorted(sizes.tolist(), reverse=True) + [0, 0, 0]
+        if k or fr != 3:
+            expect.append((fr, k, top[0], top[1], top[2]))
+    w.finalize(6)
+    w.close()
+    path = tmp_path / "stats.csv"
+    chunk = next(stats.read_stats(path, chunksize=1000))
+    assert list(chunk.columns) == list(stats.DTYPES)
+    for c, t in stats.DTYPES.items():
+        assert chunk[c].dtype == t, c
+    tables = [stats.per_frame_summary(path, chunksize=cs) for cs in (1, 2, 3, 7, 1000)]
+    for tb in tables[1:]:
+        assert tb.equals(tables[0])
+    got = [tuple(int(v) for v in r) for r in
+           tables[0][["frame", "n_agg", "max_mass", "sec_mass", "third_mass"]].to_numpy()]
+    assert got == expect
+    assert np.allclose(tables[0]["yrs"], tables[0]["t"] * 0.15915494)
+    frames = [f for f, _ in stats.iter_frames(path, chunksize=2)]
+    assert frames == sorted(set(frames))
+    with pytest.raises(ValueError, match="unknown"):
+        next(stats.read_stats(path, columns=["frame", "nope"]))
+    (tmp_path / "bad.csv").write_text("a,b\n1,2\n")
+    with pytest.raises(ValueError, match="not a stats.csv"):
+        next(stats.read_stats(tmp_path / "bad.csv"))
