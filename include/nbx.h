@@ -224,6 +224,16 @@ int nbx_tree_set_radix_bits(nbx_tree *t, int bits);
  * of a wave split each batch of tree records over 64/targets slots).  Results
  * depend on the shape only through the summation order. */
 int nbx_tree_set_walk_shape(nbx_tree *t, int targets_per_wave);
+/* Launch order of the gravity walk's wavefronts (results do not depend on
+ * it): NBX_WAVES_LAUNCH = Morton order; NBX_WAVES_LONGEST_FIRST = the
+ * stepping walk starts the waves that cost most in the previous step first;
+ * NBX_WAVES_XCD (default) = each of the 8 accelerator dies (XCDs) takes one
+ * contiguous Morton range of waves, so the tree records a die's waves read
+ * stay in that die's L2. */
+#define NBX_WAVES_LAUNCH 0
+#define NBX_WAVES_LONGEST_FIRST 1
+#define NBX_WAVES_XCD 2
+int nbx_tree_set_wave_order(nbx_tree *t, int order);
 int nbx_tree_get_walk_shape(nbx_tree *t, int32_t *targets_per_wave);
 /* Times the gravity walk of every shape on the last summarized tree (outputs
  * go to scratch, the state is not touched), keeps the fastest, and writes

@@ -77,9 +77,11 @@ struct nbx_tree {
     // side stream of the collision pass, which overlaps the force walk
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // longest-first wave order of the stepping walk; lpt_tg: the walk shape
-    // its buffers hold
-    bool lpt = true;
+    // launch order of the walk's waves (NBX_WAVES_*): longest first from the
+    // last step's costs (the stepping walk), or contiguous Morton ranges per
+    // XCD; lpt_tg: the walk shape the longest-first buffers hold
+    int wave_order = NBX_WAVES_XCD;
+    bool lpt = false;
     int lpt_tg = 0;
     bool adjoint = false;  // builds record sorted ranges; reverse-mode buffers exist
     double *h_adj = nullptr;  // pinned copy of B.adj_p
@@ -594,6 +596,21 @@ int nbx_tree_set_radix_bits(nbx_tree *t, int bits)
     return NBX_OK;
 }
 
+int nbx_tree_set_wave_order(nbx_tree *t, int order)
+{
+    if (!t) return fail(NBX_E_ARG, "null tree");
+    if (order != NBX_WAVES_LAUNCH && order != NBX_WAVES_LONGEST_FIRST && order != NBX_WAVES_XCD)
+        return fail(NBX_E_ARG, "wave order %d not in {0, 1, 2}", order);
+    t->wave_order = order;
+    t->lpt = order == NBX_WAVES_LONGEST_FIRST;
+    t->lpt_tg = 0;  // longest-first buffers restart from the identity when used again
+    if (t->graph) {
+        HIPCHK(hipGraphExecDestroy(t->graph));
+        t->graph = nullptr;
+    }
+    return NBX_OK;
+}
+
 int nbx_tree_set_walk_shape(nbx_tree *t, int targets_per_wave)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
@@ -624,7 +641,8 @@ int nbx_tune_walk(nbx_tree *t, float theta, float softening, float star_mass, fl
     float best_ms = 1e30f;
     for (int q = 0; q < 4; ++q) {
         WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad,
-                   kShapes[q], sc, sc + n, sc + 2 * n, nullptr, nullptr, nullptr};
+                   kShapes[q], sc, sc + n, sc + 2 * n, nullptr, nullptr, nullptr, 0,
+                   t->wave_order == NBX_WAVES_XCD ? 1 : 0};
         launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);  // warm-up
         HIPCHK(hipEventRecord(e0, t->stream));
         for (int rep = 0; rep < 3; ++rep) launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);
@@ -727,7 +745,7 @@ int nbx_force_ex(nbx_tree *t, nbx_particles *p, float theta, float softening, fl
     if (!(theta > 0.f)) return fail(NBX_E_ARG, "theta must be positive");
     WalkArgs W{theta * theta, softening * softening, softening, star_mass, t->B.quad, t->walk_tg,
                p->ax, p->ay,
-               p->az, self_ax, self_ay, self_az};
+               p->az, self_ax, self_ay, self_az, 0, t->wave_order == NBX_WAVES_XCD ? 1 : 0};
     launch_walk(t->T, nullptr, nullptr, W, t->B, t->stream);
     return finish(t);
 }
@@ -943,7 +961,7 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
     WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
                sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr,
-               t->lpt ? 1 : 0};
+               t->lpt ? 1 : 0, t->wave_order == NBX_WAVES_XCD ? 1 : 0};
     const bool overlap = !t->timing;
     // the next walk's wave order, from the last walk's wave costs: counted by
     // the kick-drift pass, placed by spare blocks of the key pass

@@ -23,6 +23,7 @@ constexpr int kMaxSeg = 16;      // disks per batched tree
 constexpr int kLevels = 21;      // octant digits per key (63 bits)
 constexpr int kBlock = 256;      // threads per block for streaming kernels
 constexpr int kWave = 64;        // gfx950 wavefront
+constexpr int kXcds = 8;         // MI355X accelerator dies; workgroups go to them round robin
 constexpr int kStackCap = 384;   // gravity-walk stack entries per wave (LDS)
 constexpr int kRecPad = 16;      // records of padding after the node array
 

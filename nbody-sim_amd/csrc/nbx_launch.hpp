@@ -158,6 +158,7 @@ struct WalkArgs {
     float *ax, *ay, *az;          // original order, written with '='
     float *sax, *say, *saz;       // optional self-gravity outputs
     int lpt = 0;                  // 1: launch waves longest-first (needs step_ctr)
+    int xcd = 0;                  // 1: contiguous wave ranges per XCD (overrides lpt)
 };
 // Perturber values per disk come from dtab[*step_ctr][disk] when dtab is
 // non-null (graph-replayable), otherwise no perturber.

@@ -16,21 +16,33 @@
 namespace nbx {
 
 template <int TG, bool QUAD, int U, bool SELF>
-__global__ __launch_bounds__(kWalkBlock) void k_walk(
+__global__ __launch_bounds__(kGravBlock) void k_walk(
     SegTable T, WaveMap M, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
     const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
     float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
-    float *saz, int *err, const int *__restrict__ order, int *__restrict__ cost, int lpt_stride)
+    float *saz, int *err, const int *__restrict__ order, int *__restrict__ cost, int lpt_stride,
+    int xcd_chunk)
 {
     using Lds = std::conditional_t<QUAD, WaveLdsQT<(TG > 32)>, WaveLdsT<(TG > 32)>>;
-    __shared__ Lds lds[kWalkWaves];
+    __shared__ Lds lds[kGravWaves];
     const int wib = threadIdx.x >> 6;
-    const int slot = blockIdx.x * kWalkWaves + wib;
-    if (slot >= M.wave[T.nseg]) return;  // uniform per wave
-    // launch slot -> target group: longest waves of the previous step first
+    const int slot = blockIdx.x * kGravWaves + wib;
+    // launch slot -> target group: longest waves of the previous step first,
+    // or (xcd_chunk > 0) contiguous Morton ranges per XCD: workgroups are
+    // dispatched to the 8 XCDs round robin, so block % 8 is the XCD and its
+    // waves share the tree records of one region in that XCD's L2
+    int w;
+    if (xcd_chunk > 0) {
+        const int j = (blockIdx.x / kXcds) * kGravWaves + wib;
+        if (j >= xcd_chunk) return;
+        w = (blockIdx.x % kXcds) * xcd_chunk + j;
+        if (w >= M.wave[T.nseg]) return;  // uniform per wave
+    } else {
+        if (slot >= M.wave[T.nseg]) return;
+        w = order ? order[(*step_ctr & 1) * lpt_stride + slot] : slot;
+    }
     const int par = order ? (*step_ctr & 1) * lpt_stride : 0;
-    const int w = order ? order[par + slot] : slot;
     const int s = seg_of_wmap(T, M, w);
     const int lane = __lane_id(), tt = lane % TG, rr = lane / TG;
     const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
@@ -70,14 +82,14 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(
 }
 
 template <int TG>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_counts(
+__global__ __launch_bounds__(kGravBlock) void k_walk_counts(
     SegTable T, WaveMap M, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
     const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, int *n_acc,
     int *n_open, int *err, int *wstat)
 {
-    __shared__ WaveLdsT<(TG > 32)> lds[kWalkWaves];
+    __shared__ WaveLdsT<(TG > 32)> lds[kGravWaves];
     const int wib = threadIdx.x >> 6;
-    const int w = blockIdx.x * kWalkWaves + wib;
+    const int w = blockIdx.x * kGravWaves + wib;
     if (w >= M.wave[T.nseg]) return;
     const int s = seg_of_wmap(T, M, w);
     const int lane = __lane_id(), tt = lane % TG, rr = lane / TG;
@@ -537,12 +549,14 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     auto kern = W.eps2 >= kNoSelfTestEps2
                     ? (W.quad ? k_walk<TG, true, U, false> : k_walk<TG, false, U, false>)
                     : (W.quad ? k_walk<TG, true, U, true> : k_walk<TG, false, U, true>);
-    const bool lpt = W.lpt && step_ctr;
-    hipLaunchKernelGGL(kern, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, M, dtab,
+    const bool lpt = W.lpt && step_ctr && !W.xcd;
+    const int chunk = W.xcd ? cdiv(waves, kXcds) : 0;
+    const int blocks = W.xcd ? kXcds * cdiv(chunk, kGravWaves) : cdiv(waves, kGravWaves);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kGravBlock), 0, st, T, M, dtab,
                        step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
                        B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay,
                        W.az, W.sax, W.say, W.saz, B.err, lpt ? B.lpt_order : nullptr,
-                       lpt ? B.lpt_cost : nullptr, B.lpt_stride);
+                       lpt ? B.lpt_cost : nullptr, B.lpt_stride, chunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -592,8 +606,8 @@ static void launch_counts_tg(const SegTable &T, float theta2, int *n_acc, int *n
     M.wave[0] = 0;
     for (int s = 0; s < kMaxSeg; ++s)
         M.wave[s + 1] = M.wave[s] + (s < T.nseg ? cdiv(T.off[s + 1] - T.off[s], TG) : 0);
-    hipLaunchKernelGGL(k_walk_counts<TG>, dim3(cdiv(M.wave[T.nseg], kWalkWaves)),
-                       dim3(kWalkBlock), 0, st, T, M, B.rec_a, B.rec_b, B.bodies, B.vals[B.cur],
+    hipLaunchKernelGGL(k_walk_counts<TG>, dim3(cdiv(M.wave[T.nseg], kGravWaves)),
+                       dim3(kGravBlock), 0, st, T, M, B.rec_a, B.rec_b, B.bodies, B.vals[B.cur],
                        theta2, n_acc, n_open, B.err, wstat);
 }
 

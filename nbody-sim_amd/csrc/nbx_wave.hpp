@@ -20,6 +20,12 @@ static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 constexpr int kWalkWaves = 1;   // one wavefront per block: walks never synchronise
 constexpr int kWalkBlock = kWave * kWalkWaves;
+// The gravity walk groups four independent wavefronts per block: a block's
+// waves are placed one per SIMD of a compute unit, so every SIMD gets the
+// same number of waves (single-wave blocks left some SIMDs with 3 and
+// others with 5).  The waves never synchronise with each other.
+constexpr int kGravWaves = 4;
+constexpr int kGravBlock = kWave * kGravWaves;
 constexpr int kGStack = kStackCap;  // gravity stack entries per wave
 // Records of a popped batch are evaluated before their children are pushed,
 // and the next batch is popped while the current one is evaluated, so up to

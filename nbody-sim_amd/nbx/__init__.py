@@ -113,6 +113,7 @@ EXPORTS = [
     "nbx_particle_radius", "nbx_init_disk", "nbx_writer_open", "nbx_writer_frame",
     "nbx_writer_stats", "nbx_writer_perturber", "nbx_writer_finalize", "nbx_writer_close",
     "nbx_write_perturber_sidecar", "nbx_fof_noalloc", "nbx_fof_disks", "nbx_search_stats",
+    "nbx_tree_set_wave_order",
 ]
 
 _lib = None
@@ -145,6 +146,7 @@ def lib():
             "nbx_fof_noalloc": [P, f32, vp, vp],
             "nbx_fof_disks": [vp, P, C.POINTER(f32), vp],
             "nbx_search_stats": [vp, C.POINTER(i32), i32],
+            "nbx_tree_set_wave_order": [vp, i32],
             "nbx_init_forces": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params)],
             "nbx_step": [vp, P, C.POINTER(nbx_step_params), C.POINTER(nbx_disk_params), i32, i32],
             "nbx_step_concurrent": [C.POINTER(vp), C.POINTER(P), i32, C.POINTER(nbx_step_params),
@@ -676,6 +678,15 @@ def diagnostics(t, p, star_mass=1.0, softening=1e-3, with_self=False):
 def set_radix_bits(t, bits):
     """Digit width of the key sort's radix passes (8, 9 or 11)."""
     _check(lib().nbx_tree_set_radix_bits(t.h, int(bits)), "set_radix_bits")
+
+
+WAVES_LAUNCH, WAVES_LONGEST_FIRST, WAVES_XCD = 0, 1, 2  # NBX_WAVES_* of include/nbx.h
+
+
+def set_wave_order(t, order):
+    """Launch order of the walk's waves: WAVES_LAUNCH, WAVES_LONGEST_FIRST or
+    WAVES_XCD (the default: contiguous ranges per accelerator die)."""
+    _check(lib().nbx_tree_set_wave_order(t.h, int(order)), "set_wave_order")
 
 
 def set_walk_shape(t, targets_per_wave):

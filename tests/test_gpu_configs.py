@@ -342,3 +342,26 @@ def test_deep_clump_fof_and_collisions(gpu, ll):
     ov = np.stack([vx, vy, vz], 1)
     assert np.abs(gv - ov).max() <= 1e-5 * np.abs(ov).max()
     assert pairs > 0
+
+
+def test_wave_orders_give_identical_results(gpu):
+    """The launch order of the walk's wavefronts (Morton order, longest first,
+    contiguous ranges per XCD) changes only the schedule: forces and ten
+    stepped states are bit-identical across the three orders."""
+    nbx = gpu
+    n = 65536
+    d = disk(n, mass=3e-5, seed=88)
+    sp = _gpu_params(nbx, use_graph=True)
+    dp = nbx.disk_params(np.float32(2e-3), nbx.makePerturber(True, 10.0, 5.2, 0.0, 1.0))
+    out = []
+    for order in (nbx.WAVES_LAUNCH, nbx.WAVES_LONGEST_FIRST, nbx.WAVES_XCD):
+        _say(f"wave order {order}")
+        p = nbx.to_gpu(d)
+        t = nbx.allocateOctree(n)
+        nbx.set_wave_order(t, order)
+        nbx.init_forces(t, p, sp, dp)
+        nbx.step(t, p, sp, dp, 0, 10)
+        out.append(p.numpy())
+    for o in out[1:]:
+        for k in STATE:
+            np.testing.assert_array_equal(o[k], out[0][k], err_msg=k)
