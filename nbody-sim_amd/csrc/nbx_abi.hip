@@ -74,9 +74,6 @@ struct nbx_tree {
     bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
     int walk_tg = kDefaultTargets;  // gravity-walk targets per wavefront
     float tune_ms[4] = {0, 0, 0, 0};  // autotuner timings for 64, 32, 16, 8 targets
-    // side stream of the collision pass, which overlaps the force walk
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
     // launch order of the walk's waves (NBX_WAVES_*): longest first from the
     // last step's costs (the stepping walk), or contiguous Morton ranges per
     // XCD; lpt_tg: the walk shape the longest-first buffers hold
@@ -269,9 +266,6 @@ void destroy_handles(nbx_tree *t)
     for (hipEvent_t e : t->ev) (void)hipEventDestroy(e);
     t->ev.clear();
     if (t->owned) (void)hipStreamDestroy(t->owned);
-    if (t->side) (void)hipStreamDestroy(t->side);
-    if (t->fork) (void)hipEventDestroy(t->fork);
-    if (t->join) (void)hipEventDestroy(t->join);
     if (t->gord) (void)hipEventDestroy(t->gord);
     delete t;
 }
@@ -333,9 +327,6 @@ int create_tree(nbx_tree **out, int nseg, const int32_t *counts)
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&t->cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) e = hipStreamCreate(&t->owned);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&t->side, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->fork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&t->join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&t->gord, hipEventDisableTiming);
     t->stream = t->owned;
     if (e != hipSuccess) {
@@ -490,7 +481,6 @@ int nbx_tree_destroy(nbx_tree *t)
 {
     if (!t) return NBX_OK;
     (void)hipStreamSynchronize(t->stream);
-    (void)hipStreamSynchronize(t->side);
     free_bufs(t);
     destroy_handles(t);
     return NBX_OK;
@@ -979,36 +969,19 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);
     // The collision pass reads the tree and the velocities and writes only
-    // B.dv; the walk writes only the accelerations: without phase timing the
-    // collision pass runs on the side stream beside the walk.  It is
-    // submitted first: when the walk has more waves than the GPU holds (a
-    // batch of disks), the collision waves get slots before the walk fills
-    // them, most end within microseconds, and the few long searches (bodies
-    // near the faces of large cells) finish under the walk instead of after
-    // it.
-    // Which of the two is submitted first decides which one a replayed graph
-    // keeps on the launch queue.  When every wave of the walk is resident at
-    // once (at most 16 waves per compute unit), the walk goes first: it stays
-    // on the launch queue and the step pays no cross-queue hand-off before
-    // or after it.  A larger walk (a batch of disks) is run in several
-    // rounds of waves, and the collision pass goes first so its waves get
-    // slots before the walk fills them.
-    const bool side_collide = sp->collisions && overlap;
-    const bool walk_first = walk_waves(t->T, t->walk_tg) <= 16 * t->cus;
-    auto side_pass = [&]() {
-        (void)hipStreamWaitEvent(t->side, t->fork, 0);
-        launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->side);
-        (void)hipEventRecord(t->join, t->side);
-    };
-    if (side_collide) (void)hipEventRecord(t->fork, t->stream);
-    if (side_collide && !walk_first) side_pass();
-    launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
-    if (side_collide && walk_first) side_pass();
-    if (e0 >= 0) ev_record(t, e0 + 3);
-    if (side_collide)
-        (void)hipStreamWaitEvent(t->stream, t->join, 0);
-    else if (sp->collisions)
-        launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
+    // B.dv; the walk writes only the accelerations.  Without phase timing
+    // both run as one launch (collision blocks first, then the walk blocks),
+    // so the step has no second stream and no cross-stream hand-off.
+    if (sp->collisions && overlap) {
+        launch_walk_collide(t->T, t->B.dtab, t->B.step_ctr, W, D, p->vx, p->vy, p->vz,
+                            sp->restitution, t->B, t->stream);
+        if (e0 >= 0) ev_record(t, e0 + 3);
+    } else {
+        launch_walk(t->T, t->B.dtab, t->B.step_ctr, W, t->B, t->stream);
+        if (e0 >= 0) ev_record(t, e0 + 3);
+        if (sp->collisions)
+            launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
+    }
     if (e0 >= 0) ev_record(t, e0 + 4);
     if (t->compensated)
         launch_kick_drag_c(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,

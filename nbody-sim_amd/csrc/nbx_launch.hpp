@@ -177,6 +177,10 @@ void reset_lpt(Bufs &B, hipStream_t st);
 // Achievable FP32 TFLOP/s (kind 0: FMA chains, 1: the walk's record mix);
 // scratch: a few floats of device memory; -1 on failure.
 double probe_fp32(int kind, float *scratch, hipStream_t st);
+// The walk and the collision pass of launch_collide as one launch.
+void launch_walk_collide(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
+                         const WalkArgs &W, const DiskTable &D, const float *vx, const float *vy,
+                         const float *vz, float restitution, Bufs &B, hipStream_t st);
 // Collision deltas into B.dv (original order).  r_coll per disk from D.
 void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, const float *vy,
                     const float *vz, float restitution, Bufs &B, hipStream_t st);

@@ -15,28 +15,57 @@
 
 namespace nbx {
 
+// Everything the gravity walk reads besides the disk tables.
+struct GravArgs {
+    const DiskDev *dtab;        // per-step perturber of every disk (or null)
+    const int *step_ctr;
+    const float4 *ra;
+    const int4 *rb;
+    const float4 *rq;
+    const float4 *bodies;
+    const int *perm;
+    float theta2, eps2, softening, star_mass;
+    float *ax, *ay, *az, *sax, *say, *saz;
+    int *err;
+    const int *order;           // longest-first wave order (or null)
+    int *cost;
+    int lpt_stride;
+    int xcd_chunk;              // > 0: contiguous wave ranges per XCD
+};
+
+template <int TG, bool QUAD>
+using GravLds = std::conditional_t<QUAD, WaveLdsQT<(TG > 32)>, WaveLdsT<(TG > 32)>>;
+
+// The four waves of walk block `block`.
 template <int TG, bool QUAD, int U, bool SELF>
-__global__ __launch_bounds__(kGravBlock) void k_walk(
-    SegTable T, WaveMap M, const DiskDev *__restrict__ dtab, const int *__restrict__ step_ctr,
-    const float4 *__restrict__ ra, const int4 *__restrict__ rb, const float4 *__restrict__ rq,
-    const float4 *__restrict__ bodies, const int *__restrict__ perm, float theta2, float eps2,
-    float softening, float star_mass, float *ax, float *ay, float *az, float *sax, float *say,
-    float *saz, int *err, const int *__restrict__ order, int *__restrict__ cost, int lpt_stride,
-    int xcd_chunk)
+__device__ inline void walk_block(const SegTable &T, const WaveMap &M, const GravArgs &A,
+                                  int block, GravLds<TG, QUAD> *lds)
 {
-    using Lds = std::conditional_t<QUAD, WaveLdsQT<(TG > 32)>, WaveLdsT<(TG > 32)>>;
-    __shared__ Lds lds[kGravWaves];
+    const DiskDev *__restrict__ dtab = A.dtab;
+    const int *__restrict__ step_ctr = A.step_ctr;
+    const float4 *__restrict__ ra = A.ra;
+    const int4 *__restrict__ rb = A.rb;
+    const float4 *__restrict__ rq = A.rq;
+    const float4 *__restrict__ bodies = A.bodies;
+    const int *__restrict__ perm = A.perm;
+    const float theta2 = A.theta2, eps2 = A.eps2, softening = A.softening,
+                star_mass = A.star_mass;
+    float *ax = A.ax, *ay = A.ay, *az = A.az, *sax = A.sax, *say = A.say, *saz = A.saz;
+    int *err = A.err;
+    const int *__restrict__ order = A.order;
+    int *__restrict__ cost = A.cost;
+    const int lpt_stride = A.lpt_stride, xcd_chunk = A.xcd_chunk;
     const int wib = threadIdx.x >> 6;
-    const int slot = blockIdx.x * kGravWaves + wib;
+    const int slot = block * kGravWaves + wib;
     // launch slot -> target group: longest waves of the previous step first,
     // or (xcd_chunk > 0) contiguous Morton ranges per XCD: workgroups are
     // dispatched to the 8 XCDs round robin, so block % 8 is the XCD and its
     // waves share the tree records of one region in that XCD's L2
     int w;
     if (xcd_chunk > 0) {
-        const int j = (blockIdx.x / kXcds) * kGravWaves + wib;
+        const int j = (block / kXcds) * kGravWaves + wib;
         if (j >= xcd_chunk) return;
-        w = (blockIdx.x % kXcds) * xcd_chunk + j;
+        w = (block % kXcds) * xcd_chunk + j;
         if (w >= M.wave[T.nseg]) return;  // uniform per wave
     } else {
         if (slot >= M.wave[T.nseg]) return;
@@ -81,6 +110,13 @@ __global__ __launch_bounds__(kGravBlock) void k_walk(
     }
 }
 
+template <int TG, bool QUAD, int U, bool SELF>
+__global__ __launch_bounds__(kGravBlock) void k_walk(SegTable T, WaveMap M, GravArgs A)
+{
+    __shared__ GravLds<TG, QUAD> lds[kGravWaves];
+    walk_block<TG, QUAD, U, SELF>(T, M, A, blockIdx.x, lds);
+}
+
 template <int TG>
 __global__ __launch_bounds__(kGravBlock) void k_walk_counts(
     SegTable T, WaveMap M, const float4 *__restrict__ ra, const int4 *__restrict__ rb,
@@ -106,7 +142,11 @@ __global__ __launch_bounds__(kGravBlock) void k_walk_counts(
     if (n_open) n_open[i] = no;
 }
 
-constexpr int kLStack = 32;  // per-lane stack entries of the searches (LDS)
+constexpr int kLStack = 32;   // per-lane stack entries of the FoF searches (LDS)
+// Per-lane stack entries of the collision searches, alone or inside the walk
+// launch (search radii are tiny: a stack of 12 blocks holds every search
+// but those below dense clumps, which go on without it).
+constexpr int kLStackC = 12;
 
 // Tree arrays of the per-lane searches: octree parents (binary index) of
 // octree nodes and of body leaves, every octree node's child block and the
@@ -234,12 +274,12 @@ __device__ inline void lane_walk_subtree(const float4 *__restrict__ rc,
 }
 
 // Depth-first search of the records below one children block {first,
-// count} on a per-lane stack of kLStack blocks in LDS: a block is scanned
+// count} on a per-lane stack of CAP blocks in LDS: a block is scanned
 // whole (one round trip per 8 records) and the near internal records among
 // its records are pushed.  A lane whose stack is full (a clump with every
 // cell in range: up to 7 pending blocks per level) searches that record's
 // subtree at once without a stack, so no depth or clump is out of reach.
-template <class Pair>
+template <int CAP, class Pair>
 __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
                                        const SearchTree &S, int first0, int count0, float tx,
                                        float ty, float tz, float radius, float guard, int2 *st,
@@ -266,7 +306,7 @@ __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4
                 if (B[k].w == 0) {
                     pair(Cc[k], B[k].z);
                 } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
-                    if (sp < kLStack) {
+                    if (sp < CAP) {
                         st[sp * kWave] = make_int2(B[k].z, B[k].w);
                         ++sp;
                     } else {
@@ -313,42 +353,86 @@ struct CollidePair {
 
 // Pairs of the target at sorted position pos within `radius`: the search
 // below the lowest cell that holds them all.
-template <class Pair>
+template <int CAP, class Pair>
 __device__ inline void lane_search(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
                                    const SearchTree &S, int pos, float tx, float ty, float tz,
                                    float radius, float guard, int2 *st, int *err, Pair &pair)
 {
     const int j = search_start(S.opar, S.opar_leaf, S.bin_cell, pos, tx, ty, tz, radius, guard);
     if (j < 0) return;  // a disk of one body has no pairs
-    lane_walk_block(rc, rb, S, S.first[j], S.cnt[j], tx, ty, tz, radius, guard, st, err, pair);
+    lane_walk_block<CAP>(rc, rb, S, S.first[j], S.cnt[j], tx, ty, tz, radius, guard, st, err, pair);
 }
 
-__global__ __launch_bounds__(kWalkBlock) void k_collide(
-    SegTable T, DiskTable D, const float4 *__restrict__ rc, const int4 *__restrict__ rb,
-    const float4 *__restrict__ bodies, const int *__restrict__ perm,
-    const float *__restrict__ vx, const float *__restrict__ vy, const float *__restrict__ vz,
-    const Root *__restrict__ roots, float restitution, float *dv, int *err, SearchTree S)
+// Collision deltas of wave w (64 consecutive sorted bodies); st: the lane's
+// stack of CAP entries (stride kWave).
+struct CollideArgs {
+    DiskTable D;
+    const float4 *rc;
+    const float *vx, *vy, *vz;
+    const Root *roots;
+    float restitution;
+    float *dv;
+};
+
+template <int CAP>
+__device__ inline void collide_wave(const SegTable &T, const CollideArgs &C,
+                                    const int4 *__restrict__ rb, const float4 *__restrict__ bodies,
+                                    const int *__restrict__ perm, int *err, const SearchTree &S,
+                                    int w, int2 *st)
 {
-    __shared__ int2 st[kLStack * kWalkBlock];
-    const int wib = threadIdx.x >> 6;
-    const int w = blockIdx.x * kWalkWaves + wib;
-    if (w >= T.wave[T.nseg]) return;
     const int s = seg_of_wave(T, w);
     const int pos = T.off[s] + (w - T.wave[s]) * kWave + __lane_id();
     if (pos >= T.off[s + 1]) return;
     const float4 b = bodies[pos];
     const int i = perm[pos];
-    const float r = D.d[s].r_coll;
-    CollidePair v{b.x, b.y, b.z, r * r, restitution, vx[i], vy[i], vz[i], pos, perm, vx, vy, vz,
-                  0.f, 0.f, 0.f, 0};
+    const float r = C.D.d[s].r_coll;
+    CollidePair v{b.x, b.y, b.z, r * r, C.restitution, C.vx[i], C.vy[i], C.vz[i], pos, perm,
+                  C.vx, C.vy, C.vz, 0.f, 0.f, 0.f, 0};
     if (r > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(roots[s]),
-                    st + wib * kLStack * kWave + __lane_id(), err, v);
+        lane_search<CAP>(C.rc, rb, S, pos, b.x, b.y, b.z, r, root_guard(C.roots[s]), st, err, v);
     const int n = T.n_total;
-    dv[i] = v.dvx;
-    dv[n + i] = v.dvy;
-    dv[2 * n + i] = v.dvz;
+    C.dv[i] = v.dvx;
+    C.dv[n + i] = v.dvy;
+    C.dv[2 * n + i] = v.dvz;
     if (v.pairs) atomicAdd(err + 1, v.pairs);
+}
+
+__global__ __launch_bounds__(kWalkBlock) void k_collide(SegTable T, CollideArgs C,
+                                                        const int4 *__restrict__ rb,
+                                                        const float4 *__restrict__ bodies,
+                                                        const int *__restrict__ perm, int *err,
+                                                        SearchTree S)
+{
+    __shared__ int2 st[kLStackC * kWalkBlock];
+    const int wib = threadIdx.x >> 6;
+    const int w = blockIdx.x * kWalkWaves + wib;
+    if (w >= T.wave[T.nseg]) return;
+    collide_wave<kLStackC>(T, C, rb, bodies, perm, err, S, w,
+                           st + wib * kLStackC * kWave + __lane_id());
+}
+
+// The step's walk and collision pass as one launch: blocks [0, ncb) compute
+// the collision deltas (four waves of 64 bodies each, stacks of kLStackC
+// entries in the walk's LDS), the rest are walk blocks.  The collision blocks
+// come first, so they start first; most of their waves end within
+// microseconds and the walk fills the compute units; no second stream and
+// no cross-stream hand-off before or after the walk.
+template <int TG, bool QUAD, int U, bool SELF>
+__global__ __launch_bounds__(kGravBlock) void k_walk_collide(SegTable T, WaveMap M, GravArgs A,
+                                                             int ncb, CollideArgs C, SearchTree S)
+{
+    __shared__ GravLds<TG, QUAD> lds[kGravWaves];
+    static_assert(sizeof(lds) >= sizeof(int2) * kLStackC * kGravBlock,
+                  "the collision stacks fit in the walk's LDS");
+    if ((int)blockIdx.x < ncb) {
+        const int wib = threadIdx.x >> 6;
+        const int w = blockIdx.x * kGravWaves + wib;
+        if (w >= T.wave[T.nseg]) return;
+        int2 *st = reinterpret_cast<int2 *>(lds) + wib * kLStackC * kWave + __lane_id();
+        collide_wave<kLStackC>(T, C, A.rb, A.bodies, A.perm, A.err, S, w, st);
+        return;
+    }
+    walk_block<TG, QUAD, U, SELF>(T, M, A, blockIdx.x - ncb, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -423,7 +507,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_fof_link(SegTable T, const float
     const float ll = LT.ll[s];  // the disk's own link length
     FofPair v{b.x, b.y, b.z, ll * ll, perm[pos], perm, parent};
     if (ll > 0.f)
-        lane_search(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]),
+        lane_search<kLStack>(rc, rb, S, pos, b.x, b.y, b.z, ll, root_guard(roots[s]),
                     st + wib * kLStack * kWave + __lane_id(), err, v);
 }
 
@@ -534,9 +618,15 @@ double probe_fp32(int kind, float *scratch, hipStream_t st)
 
 constexpr int kUnroll64 = 2;
 
+static CollideArgs collide_args(const DiskTable &D, const float *vx, const float *vy,
+                               const float *vz, float restitution, Bufs &B)
+{
+    return CollideArgs{D, B.rec_c, vx, vy, vz, B.roots, restitution, B.dv};
+}
+
 template <int TG>
 static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
-                           const WalkArgs &W, Bufs &B, hipStream_t st)
+                           const WalkArgs &W, Bufs &B, hipStream_t st, const CollideArgs *C)
 {
     WaveMap M;
     M.wave[0] = 0;
@@ -545,18 +635,30 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     const int waves = M.wave[T.nseg];
     // with one target per lane a lane evaluates kUnroll64 records per step
     constexpr int U = TG >= 16 ? kUnroll64 : 1;
-    // the own-leaf test is needed only without softening
-    auto kern = W.eps2 >= kNoSelfTestEps2
-                    ? (W.quad ? k_walk<TG, true, U, false> : k_walk<TG, false, U, false>)
-                    : (W.quad ? k_walk<TG, true, U, true> : k_walk<TG, false, U, true>);
     const bool lpt = W.lpt && step_ctr && !W.xcd;
     const int chunk = W.xcd ? cdiv(waves, kXcds) : 0;
     const int blocks = W.xcd ? kXcds * cdiv(chunk, kGravWaves) : cdiv(waves, kGravWaves);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kGravBlock), 0, st, T, M, dtab,
-                       step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
-                       B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay,
-                       W.az, W.sax, W.say, W.saz, B.err, lpt ? B.lpt_order : nullptr,
-                       lpt ? B.lpt_cost : nullptr, B.lpt_stride, chunk);
+    GravArgs A{dtab, step_ctr, B.rec_a, B.rec_b, W.quad ? B.rec_q : nullptr, B.bodies,
+               B.vals[B.cur], W.theta2, W.eps2, W.softening, W.star_mass, W.ax, W.ay, W.az,
+               W.sax, W.say, W.saz, B.err, lpt ? B.lpt_order : nullptr,
+               lpt ? B.lpt_cost : nullptr, B.lpt_stride, chunk};
+    // the own-leaf test is needed only without softening
+    const bool self = W.eps2 < kNoSelfTestEps2;
+    if (!C) {
+        auto kern = !self ? (W.quad ? k_walk<TG, true, U, false> : k_walk<TG, false, U, false>)
+                          : (W.quad ? k_walk<TG, true, U, true> : k_walk<TG, false, U, true>);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kGravBlock), 0, st, T, M, A);
+        return;
+    }
+    // collision blocks: a multiple of the XCD count, so the walk blocks keep
+    // their die assignment
+    const int ncb = kXcds * cdiv(cdiv(T.wave[T.nseg], kGravWaves), kXcds);
+    auto kern = !self ? (W.quad ? k_walk_collide<TG, true, U, false>
+                                : k_walk_collide<TG, false, U, false>)
+                      : (W.quad ? k_walk_collide<TG, true, U, true>
+                                : k_walk_collide<TG, false, U, true>);
+    hipLaunchKernelGGL(kern, dim3(ncb + blocks), dim3(kGravBlock), 0, st, T, M, A, ncb, *C,
+                       search_bufs(B));
 }
 
 // ---------------------------------------------------------------------------
@@ -587,15 +689,29 @@ LptDev lpt_dev(Bufs &B, int nw, const int *step_ctr)
     return L;
 }
 
+static void launch_walk_any(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
+                            const WalkArgs &W, Bufs &B, hipStream_t st, const CollideArgs *C)
+{
+    switch (W.targets) {
+    case 8: launch_walk_tg<8>(T, dtab, step_ctr, W, B, st, C); break;
+    case 16: launch_walk_tg<16>(T, dtab, step_ctr, W, B, st, C); break;
+    case 32: launch_walk_tg<32>(T, dtab, step_ctr, W, B, st, C); break;
+    default: launch_walk_tg<64>(T, dtab, step_ctr, W, B, st, C); break;
+    }
+}
+
 void launch_walk(const SegTable &T, const DiskDev *dtab, const int *step_ctr, const WalkArgs &W,
                  Bufs &B, hipStream_t st)
 {
-    switch (W.targets) {
-    case 8: launch_walk_tg<8>(T, dtab, step_ctr, W, B, st); break;
-    case 16: launch_walk_tg<16>(T, dtab, step_ctr, W, B, st); break;
-    case 32: launch_walk_tg<32>(T, dtab, step_ctr, W, B, st); break;
-    default: launch_walk_tg<64>(T, dtab, step_ctr, W, B, st); break;
-    }
+    launch_walk_any(T, dtab, step_ctr, W, B, st, nullptr);
+}
+
+void launch_walk_collide(const SegTable &T, const DiskDev *dtab, const int *step_ctr,
+                         const WalkArgs &W, const DiskTable &D, const float *vx, const float *vy,
+                         const float *vz, float restitution, Bufs &B, hipStream_t st)
+{
+    const CollideArgs C = collide_args(D, vx, vy, vz, restitution, B);
+    launch_walk_any(T, dtab, step_ctr, W, B, st, &C);
 }
 
 template <int TG>
@@ -633,9 +749,9 @@ void launch_collide(const SegTable &T, const DiskTable &D, const float *vx, cons
                     const float *vz, float restitution, Bufs &B, hipStream_t st)
 {
     const int waves = T.wave[T.nseg];
-    hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T, D,
-                       B.rec_c, B.rec_b, B.bodies, B.vals[B.cur], vx, vy, vz, B.roots, restitution,
-                       B.dv, B.err, search_bufs(B));
+    hipLaunchKernelGGL(k_collide, dim3(cdiv(waves, kWalkWaves)), dim3(kWalkBlock), 0, st, T,
+                       collide_args(D, vx, vy, vz, restitution, B), B.rec_b, B.bodies,
+                       B.vals[B.cur], B.err, search_bufs(B));
 }
 
 void launch_fof(const SegTable &T, const LinkTable &LT, int *labels, Bufs &B, hipStream_t st)
