@@ -746,11 +746,14 @@ def force_accuracy(nbx, t, p, n, n_targets=1024):
 
 
 def precision_study(nbx, np, torch, make_disk, args):
-    """Force-only precision study on a 1,048,576-body disk (r 0.3-10 AU):
-    walk error against FP64 for theta 0.3 / 0.5 / 0.7 on 16,384 seeded
-    targets, and the drift of the FP64 energy (kinetic + star potential) and
-    angular momentum over 500 gravity-only steps with the float KDK and with
-    the compensated FP64 KDK (forces FP32 in both)."""
+    """Precision study on a 1,048,576-body disk (r 0.3-10 AU): walk error
+    against the FP64 direct sum for theta 0.3 / 0.5 / 0.7 on 16,384 seeded
+    targets; then 500 gravity-only steps (theta 0.5, star + self-gravity) of
+    the same initial state with the float step, the compensated KDK (forces
+    FP32) and the all-FP64 step (the reference): each run's drift of the FP64
+    energy (kinetic + star + pairwise self potential) and of L_y, and the
+    float and compensated runs' position / velocity error against the FP64
+    run."""
     n = 1 << 20
     d = make_disk(n, DISK_MASS, 0.3, 10.0, seed=99)
     p = nbx.to_gpu(d)
@@ -766,27 +769,65 @@ def precision_study(nbx, np, torch, make_disk, args):
         nbx.launchSummarizeKernel(t)
         e["walk_ms"] = nbx.tune_walk(t, th, 1e-3, 1.0)[nbx.walk_shape(t)]
         out["theta"][str(th)] = e
+    t.free()
     sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, gas_drag_rate=0.0,
                          collisions=False, use_graph=True)
     dp = nbx.disk_params(0.0, None)
     steps = args.precision_steps
-    for mode in ("float", "compensated"):
+    final = {}
+    e_start = None
+    for mode in ("fp64", "float", "compensated"):
         q = nbx.to_gpu(d)
         t2 = nbx.allocateOctree(n)
-        nbx.set_precision(t2, mode == "compensated")
+        nbx.set_precision(t2, mode)
         nbx.init_forces(t2, q, sp, dp)
-        d0 = nbx.diagnostics(t2, q)
+        d0 = nbx.diagnostics(t2, q, with_self=e_start is None)
+        if e_start is None:  # the same initial state for every mode
+            w0 = d0["self_potential"]
+            e_start = d0["kinetic"] + d0["star_potential"] + w0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         nbx.step(t2, q, sp, dp, 0, steps)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
-        d1 = nbx.diagnostics(t2, q)
-        e0, e1 = d0["kinetic"] + d0["star_potential"], d1["kinetic"] + d1["star_potential"]
+        d1 = nbx.diagnostics(t2, q, with_self=True)
+        e1 = d1["kinetic"] + d1["star_potential"] + d1["self_potential"]
+        e0_ks = d0["kinetic"] + d0["star_potential"]
+        final[mode] = nbx.state64(t2, q)[:6].clone()
         out[mode] = {"steps": steps, "ms_per_step": ms,
-                     "energy_rel_drift": (e1 - e0) / abs(e0),
+                     "energy_rel_drift": (e1 - e_start) / abs(e_start),
+                     "energy_without_self_rel_drift":
+                         (d1["kinetic"] + d1["star_potential"] - e0_ks) / abs(e0_ks),
+                     "self_potential_change": d1["self_potential"] - w0,
                      "Ly_rel_drift": (d1["L"][1] - d0["L"][1]) / abs(d0["L"][1])}
         t2.free()
+        del q
+    # the FP64 run again at dt / 2 over the same time: a drift that falls as
+    # dt^2 is the integrator's truncation error, not rounding
+    sp2 = nbx.step_params(dt=0.005, theta=0.5, softening=1e-3, star_mass=1.0, gas_drag_rate=0.0,
+                          collisions=False, use_graph=True)
+    q = nbx.to_gpu(d)
+    t2 = nbx.allocateOctree(n)
+    nbx.set_precision(t2, "fp64")
+    nbx.init_forces(t2, q, sp2, dp)
+    nbx.step(t2, q, sp2, dp, 0, 2 * steps)
+    d1 = nbx.diagnostics(t2, q, with_self=True)
+    out["fp64_half_dt"] = {
+        "steps": 2 * steps, "dt": 0.005,
+        "energy_rel_drift":
+            (d1["kinetic"] + d1["star_potential"] + d1["self_potential"] - e_start) / abs(e_start)}
+    t2.free()
+    del q
+    ref64 = final["fp64"]
+    for mode in ("float", "compensated"):
+        dx = (final[mode][:3] - ref64[:3]).norm(dim=0)
+        dv = (final[mode][3:] - ref64[3:]).norm(dim=0)
+        out[mode]["vs_fp64"] = {
+            "pos_err_rms_au": float(dx.square().mean().sqrt()),
+            "pos_err_max_au": float(dx.max()),
+            "vel_err_rms": float(dv.square().mean().sqrt()),
+            "energy_minus_fp64": out[mode]["energy_rel_drift"] - out["fp64"]["energy_rel_drift"]}
+    out["energy"] = "kinetic + softened star potential + softened pairwise self potential (FP64)"
     return out
 
 

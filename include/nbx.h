@@ -241,12 +241,30 @@ int nbx_tree_get_walk_shape(nbx_tree *t, int32_t *targets_per_wave);
 int nbx_tune_walk(nbx_tree *t, float theta, float softening, float star_mass, float *ms_out);
 
 /* ---- precision (no reference counterpart) -------------------------------- */
-/* compensated=1: the KDK position / velocity updates (and the drag) of
- * nbx_step are evaluated in FP64 on hi + lo, the caller's float arrays hold
- * the rounded values and the tree holds the low-order parts; forces stay in
- * FP32.  Enabling clears the low parts (the state is its float values). */
-int nbx_tree_set_precision(nbx_tree *t, int compensated);
-/* FP64 sums over the state (low parts included in compensated mode):
+/* NBX_PRECISION_FLOAT (default): the reference's float step.
+ * NBX_PRECISION_COMPENSATED: the KDK position / velocity updates (and the
+ * drag) of nbx_step are evaluated in FP64 on hi + lo, the caller's float
+ * arrays hold the rounded values and the tree holds the low-order parts;
+ * forces stay in FP32.  Enabling clears the low parts (the state is its
+ * float values).
+ * NBX_PRECISION_FP64: the FP64 step, the reference of the precision study:
+ * the tree holds the state in FP64; KDK, drag, node moments, the walk (same
+ * octree and acceptance rule), star and perturber terms are FP64 operations;
+ * the caller's float arrays receive the rounded state (positions,
+ * velocities, accelerations).  nbx_init_forces loads the FP64 state from the
+ * float arrays; nbx_step then advances it.  No collision pass, monopoles
+ * only (NBX_E_ARG otherwise); at most 2^27 walk records. */
+#define NBX_PRECISION_FLOAT 0
+#define NBX_PRECISION_COMPENSATED 1
+#define NBX_PRECISION_FP64 2
+int nbx_tree_set_precision(nbx_tree *t, int mode);
+/* No reference counterpart: the state as doubles into the device array
+ * dev_out[9 n] = [x | y | z | vx | vy | vz | ax | ay | az] (body order): the
+ * FP64 state, hi + lo in compensated mode (accelerations float), else the
+ * float state. */
+int nbx_state64(nbx_tree *t, const nbx_particles *p, double *dev_out);
+/* FP64 sums over the state (low parts included in compensated mode, the
+ * FP64 state in FP64 mode):
  * out[0] kinetic energy, out[1] softened star potential, out[2..4] angular
  * momentum, out[5] softened pairwise self potential (O(n^2), only if
  * with_self, else 0), out[6] total mass.  out is a host array of 7. */

@@ -71,7 +71,10 @@ struct nbx_tree {
     // captured single step: replayed once per step with the device step counter
     hipGraphExec_t graph = nullptr;
     std::vector<unsigned char> graph_key;
-    bool compensated = false;  // FP64-compensated KDK (B.lo holds the low parts)
+    // NBX_PRECISION_*: float step, FP64-compensated KDK (B.lo holds the low
+    // parts) or FP64 step (B.s64 holds the state once s64_ready)
+    int precision = NBX_PRECISION_FLOAT;
+    bool s64_ready = false;
     int walk_tg = kDefaultTargets;  // gravity-walk targets per wavefront
     float tune_ms[4] = {0, 0, 0, 0};  // autotuner timings for 64, 32, 16, 8 targets
     // launch order of the walk's waves (NBX_WAVES_*): longest first from the
@@ -235,7 +238,7 @@ void free_bufs(nbx_tree *t)
                     B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
                     B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay,
-                    B.kids,    B.lpt_hist};
+                    B.kids,    B.lpt_hist, B.s64,  B.rec64};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (t->h_err) (void)hipHostFree(t->h_err);
@@ -257,6 +260,14 @@ int check_tree_particles(nbx_tree *t, const nbx_particles *p)
     if (!particles_ok(p)) return fail(NBX_E_ARG, "invalid particle arrays");
     if (p->n != t->T.n_total)
         return fail(NBX_E_ARG, "particle count %d != tree bodies %d", p->n, t->T.n_total);
+    return NBX_OK;
+}
+
+// What the FP64 step leaves out: the collision pass and quadrupoles.
+int fp64_checks(nbx_tree *t, const nbx_step_params *sp)
+{
+    if (sp->collisions) return fail(NBX_E_ARG, "the FP64 step has no collision pass");
+    if (t->B.quad) return fail(NBX_E_ARG, "the FP64 step walks monopoles only");
     return NBX_OK;
 }
 
@@ -529,12 +540,15 @@ int nbx_tree_set_multipole(nbx_tree *t, int order)
     return NBX_OK;
 }
 
-int nbx_tree_set_precision(nbx_tree *t, int compensated)
+int nbx_tree_set_precision(nbx_tree *t, int mode)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
+    if (mode != NBX_PRECISION_FLOAT && mode != NBX_PRECISION_COMPENSATED &&
+        mode != NBX_PRECISION_FP64)
+        return fail(NBX_E_ARG, "precision %d not in {0, 1, 2}", mode);
     Bufs &B = t->B;
     const int n = t->T.n_total;
-    if (compensated) {
+    if (mode == NBX_PRECISION_COMPENSATED) {
         if (!B.lo) {
             int rc = dmalloc(&B.lo, (size_t)6 * n);
             if (rc) return rc;
@@ -542,12 +556,36 @@ int nbx_tree_set_precision(nbx_tree *t, int compensated)
         HIPCHK(hipStreamSynchronize(t->stream));
         HIPCHK(hipMemset(B.lo, 0, sizeof(float) * 6 * n));  // state = its float values
     }
-    if (t->compensated != (compensated != 0) && t->graph) {
+    if (mode == NBX_PRECISION_FP64) {
+        // stack entries of the FP64 walk pack a record index above 4 bits
+        if (t->T.noff[t->T.nseg] >= (1 << 27))
+            return fail(NBX_E_ARG, "the FP64 step takes at most 2^27 walk records");
+        int rc;
+        if ((!B.s64 && (rc = dmalloc(&B.s64, (size_t)9 * n))) ||
+            (!B.rec64 && (rc = dmalloc(&B.rec64, (size_t)B.n_rec))) ||
+            (!B.brange && (rc = dmalloc(&B.brange, (size_t)n))))
+            return rc;
+        t->stage = kNone;  // the next build records node ranges
+        t->s64_ready = false;  // loaded from the float state by nbx_init_forces
+    }
+    if (t->precision != mode && t->graph) {
         HIPCHK(hipGraphExecDestroy(t->graph));
         t->graph = nullptr;
     }
-    t->compensated = compensated != 0;
+    t->precision = mode;
     return NBX_OK;
+}
+
+int nbx_state64(nbx_tree *t, const nbx_particles *p, double *dev_out)
+{
+    int rc = check_tree_particles(t, p);
+    if (rc) return rc;
+    if (!dev_out) return fail(NBX_E_ARG, "null output");
+    const bool s64 = t->precision == NBX_PRECISION_FP64 && t->s64_ready;
+    launch_state64(p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
+                   t->precision == NBX_PRECISION_COMPENSATED ? t->B.lo : nullptr,
+                   s64 ? t->B.s64 : nullptr, p->n, dev_out, t->stream);
+    return finish(t);
 }
 
 int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float softening,
@@ -560,9 +598,11 @@ int nbx_diagnostics(nbx_tree *t, const nbx_particles *p, float star_mass, float 
         rc = dmalloc(&t->B.diag, 8);
         if (rc) return rc;
     }
+    const bool s64 = t->precision == NBX_PRECISION_FP64 && t->s64_ready;
     launch_diagnostics(p->x, p->y, p->z, p->vx, p->vy, p->vz, p->mass,
-                       t->compensated ? t->B.lo : nullptr, p->n, star_mass, softening, with_self,
-                       t->B, t->stream);
+                       t->precision == NBX_PRECISION_COMPENSATED ? t->B.lo : nullptr,
+                       s64 ? t->B.s64 : nullptr, p->n, star_mass, softening, with_self, t->B,
+                       t->stream);
     HIPCHK(hipMemcpyAsync(out, t->B.diag, 7 * sizeof(double), hipMemcpyDeviceToHost, t->stream));
     HIPCHK(hipStreamSynchronize(t->stream));
     return finish(t);
@@ -930,6 +970,26 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     int rc = check_tree_particles(t, p);
     if (rc) return rc;
     if (!sp || !dp) return fail(NBX_E_ARG, "null parameters");
+    if (t->precision == NBX_PRECISION_FP64) {
+        rc = fp64_checks(t, sp);
+        if (rc) return rc;
+        // the FP64 state starts from the caller's float state
+        launch_load64(p->x, p->y, p->z, p->vx, p->vy, p->vz, t->B.s64, p->n, t->stream);
+        t->s64_ready = true;
+        launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
+        launch_root(t->T, t->B, t->stream);
+        launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, true, LptDev{},
+                     t->B.s64);
+        launch_summarize(t->T, t->B, t->stream);
+        t->stage = kSummarized;
+        rc = upload_steps(t, dp, 0, 1, sp->dt, true);
+        if (rc) return rc;
+        launch_rec64(t->T, t->B, t->stream);
+        launch_walk64(t->T, t->B.dtab, t->B.step_ctr, (double)sp->theta * (double)sp->theta,
+                      (double)sp->softening * (double)sp->softening, sp->star_mass, p->ax, p->ay,
+                      p->az, t->B, t->stream);
+        return finish(t);
+    }
     launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
     launch_root(t->T, t->B, t->stream);
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
@@ -943,9 +1003,39 @@ int nbx_init_forces(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     return finish(t);
 }
 
+// The FP64 step (no collisions, monopole walk): KDK, build and walk on the
+// FP64 state; the phases are timed like the float step's (no collision pass).
+static void enqueue_step64(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp, int e0)
+{
+    const double dt = sp->dt, dth = 0.5 * dt;
+    const int has_drag = sp->gas_drag_rate > 0.0f;
+    const double decay = has_drag ? exp(-(double)sp->gas_drag_rate * dt) : 1.0;
+    if (e0 >= 0) ev_record(t, e0 + 0);
+    launch_kick_drift_bbox_d(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, dth, dt, t->B,
+                             t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 1);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false, LptDev{},
+                 t->B.s64);
+    launch_summarize(t->T, t->B, t->stream);
+    launch_rec64(t->T, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 2);
+    launch_walk64(t->T, t->B.dtab, t->B.step_ctr, (double)sp->theta * (double)sp->theta,
+                  (double)sp->softening * (double)sp->softening, sp->star_mass, p->ax, p->ay,
+                  p->az, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 3);
+    if (e0 >= 0) ev_record(t, e0 + 4);
+    launch_kick_drag_d(p->vx, p->vy, p->vz, dth, has_drag, decay, sp->star_mass, p->n,
+                       t->B.step_ctr, t->B, t->stream);
+    if (e0 >= 0) ev_record(t, e0 + 5);
+}
+
 static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
                          const DiskTable &D, int e0)
 {
+    if (t->precision == NBX_PRECISION_FP64) {
+        enqueue_step64(t, p, sp, e0);
+        return;
+    }
     const float dt = sp->dt, dth = sp->dt * 0.5f;
     const int has_drag = sp->gas_drag_rate > 0.0f;
     const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
@@ -958,7 +1048,7 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
     const LptDev lpt = t->lpt ? lpt_dev(t->B, walk_waves(t->T, t->walk_tg), t->B.step_ctr)
                               : LptDev{};
     if (e0 >= 0) ev_record(t, e0 + 0);
-    if (t->compensated)
+    if (t->precision == NBX_PRECISION_COMPENSATED)
         launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
                                  p->az, dth, dt, t->B, t->stream, lpt);
     else
@@ -983,7 +1073,7 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
             launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
     }
     if (e0 >= 0) ev_record(t, e0 + 4);
-    if (t->compensated)
+    if (t->precision == NBX_PRECISION_COMPENSATED)
         launch_kick_drag_c(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
                            sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag,
                            has_drag ? exp(-(double)sp->gas_drag_rate * (double)sp->dt) : 1.0,
@@ -1019,6 +1109,13 @@ static std::vector<unsigned char> step_key(const nbx_particles *p, const nbx_ste
 static int prepare_steps(nbx_tree *t, const nbx_step_params *sp, const nbx_disk_params *dp,
                          int32_t step0, int32_t n_steps, DiskTable &D)
 {
+    if (t->precision == NBX_PRECISION_FP64) {
+        int rc = fp64_checks(t, sp);
+        if (rc) return rc;
+        if (!t->s64_ready)
+            return fail(NBX_E_STATE, "FP64 state not loaded: call nbx_init_forces after "
+                                     "nbx_tree_set_precision(t, NBX_PRECISION_FP64)");
+    }
     int rc = upload_steps(t, dp, step0, n_steps, sp->dt, false);
     if (rc) return rc;
     D = disk_rcoll(t, dp);
@@ -1168,7 +1265,7 @@ int nbx_tree_set_adjoint(nbx_tree *t, int enable)
     Bufs &B = t->B;
     const int n = t->T.n_total;
     int rc;
-    if ((rc = dmalloc(&B.brange, n)) || (rc = dmalloc(&B.rec_r, B.n_rec)) ||
+    if ((!B.brange && (rc = dmalloc(&B.brange, n))) || (rc = dmalloc(&B.rec_r, B.n_rec)) ||
         (rc = dmalloc(&B.adj_g, 3 * (size_t)B.n_rec)) ||
         (rc = dmalloc(&B.adj_d, 3 * (size_t)(n + 1))) ||
         (rc = dmalloc(&B.adj_blk, 3 * (size_t)adj_scan_blocks(n))) || (rc = dmalloc(&B.adj_p, 4)))
@@ -1194,7 +1291,8 @@ int adj_check(nbx_tree *t, const nbx_particles *p)
     if (!t->adjoint) return fail(NBX_E_STATE, "reverse mode not enabled (nbx_tree_set_adjoint)");
     if (t->T.nseg != 1) return fail(NBX_E_ARG, "reverse mode takes one disk per tree");
     if (t->B.quad) return fail(NBX_E_ARG, "reverse mode differentiates the monopole walk only");
-    if (t->compensated) return fail(NBX_E_ARG, "reverse mode follows the float KDK only");
+    if (t->precision != NBX_PRECISION_FLOAT)
+        return fail(NBX_E_ARG, "reverse mode follows the float KDK only");
     return NBX_OK;
 }
 

@@ -65,6 +65,8 @@ struct Bufs {
     int *labels = nullptr;       // [n] FoF scratch
     int cur = 0;                 // buffer of keys[] / vals[] holding the sorted order
     float *lo = nullptr;         // [6 n] low-order parts of x,y,z,vx,vy,vz (compensated mode)
+    double *s64 = nullptr;       // [9 n] FP64 state x,y,z,vx,vy,vz,ax,ay,az (FP64 step)
+    double4 *rec64 = nullptr;    // [n_rec] FP64 {com, mass} of walk records (FP64 step)
     double *diag = nullptr;      // [8] FP64 diagnostic sums
     DiskDev *dtab = nullptr;     // [dtab_cap * kMaxSeg] per-step disk values
     int dtab_cap = 0;
@@ -132,9 +134,35 @@ void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const 
                         int *step_ctr, Bufs &B, hipStream_t st);
 // FP64 sums: out[0] kinetic, [1] star potential, [2] L_x, [3] L_y, [4] L_z,
 // [5] self potential (pairwise, softened; only if with_self), [6] total mass.
+// The state is B.s64 when s64 is given, else the float arrays (+ lo).
 void launch_diagnostics(const float *x, const float *y, const float *z, const float *vx,
-                        const float *vy, const float *vz, const float *m, const float *lo, int n,
-                        double star_mass, double soft, int with_self, Bufs &B, hipStream_t st);
+                        const float *vy, const float *vz, const float *m, const float *lo,
+                        const double *s64, int n, double star_mass, double soft, int with_self,
+                        Bufs &B, hipStream_t st);
+
+// FP64 step: B.s64 = [x | y | z | vx | vy | vz | ax | ay | az] (9 n doubles,
+// body order); the float arrays receive the rounded values after each pass.
+void launch_load64(const float *x, const float *y, const float *z, const float *vx,
+                   const float *vy, const float *vz, double *s64, int n, hipStream_t st);
+// out = [x | y | z | vx | vy | vz | ax | ay | az] as doubles from s64,
+// hi + lo, or the floats.
+void launch_state64(const float *x, const float *y, const float *z, const float *vx,
+                    const float *vy, const float *vz, const float *ax, const float *ay,
+                    const float *az, const float *lo, const double *s64, int n, double *out,
+                    hipStream_t st);
+void launch_kick_drift_bbox_d(const SegTable &T, float *x, float *y, float *z, float *vx,
+                              float *vy, float *vz, double dt_half, double dt, Bufs &B,
+                              hipStream_t st);
+// v += a dt_half; drag (if has_drag); advances step_ctr (if given).
+void launch_kick_drag_d(float *vx, float *vy, float *vz, double dt_half, int has_drag,
+                        double decay, double star_mass, int n, int *step_ctr, Bufs &B,
+                        hipStream_t st);
+// Node moments in FP64 (B.rec64 from the FP64 prefix and the state) and the
+// FP64 walk: acceleration of every body into B.s64 (and rounded into ax..az).
+void launch_rec64(const SegTable &T, Bufs &B, hipStream_t st);
+void launch_walk64(const SegTable &T, const DiskDev *dtab, const int *step_ctr, double theta2,
+                   double eps2, double star_mass, float *ax, float *ay, float *az, Bufs &B,
+                   hipStream_t st);
 
 // ---- tree -----------------------------------------------------------------
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
@@ -142,10 +170,12 @@ void launch_bbox(const SegTable &T, const float *x, const float *y, const float 
 void launch_root(const SegTable &T, Bufs &B, hipStream_t st);
 // resample = false lets the splitter sort reuse the splitters of the previous
 // build (the step loop); every other build samples them afresh.  lpt: the
-// placement phase of the next walk's wave order (nw = 0: none).
+// placement phase of the next walk's wave order (nw = 0: none).  s64: the
+// FP64 state ([x | y | z | ...] in body order) whose positions the node
+// moments are summed from (the FP64 step); null: the float bodies.
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
                   const float *mass, Bufs &B, hipStream_t st, bool resample = true,
-                  const LptDev &lpt = LptDev{});
+                  const LptDev &lpt = LptDev{}, const double *s64 = nullptr);
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st);
 void launch_fill_int(int *p, int v, int n, hipStream_t st);
 void launch_fill_float(float *p, float v, int n, hipStream_t st);

@@ -237,6 +237,95 @@ __global__ void k_kick_drag_c(float *vx, float *vy, float *vz, const float *__re
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 step (no reference counterpart; the reference of the precision
+// study): the state lives in s64 = [x | y | z | vx | vy | vz | ax | ay | az]
+// (9 n doubles, body order) and every update is an FP64 operation; the
+// caller's float arrays receive the rounded state after each pass.
+// ---------------------------------------------------------------------------
+__global__ void k_load64(const float *__restrict__ x, const float *__restrict__ y,
+                         const float *__restrict__ z, const float *__restrict__ vx,
+                         const float *__restrict__ vy, const float *__restrict__ vz, double *s64,
+                         int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *src[6] = {x, y, z, vx, vy, vz};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s64[(size_t)k * n + i] = (double)src[k][i];
+}
+
+// The state of every body as doubles, [x | y | z | vx | vy | vz | ax | ay |
+// az]: the FP64 state (s64), hi + lo (lo; accelerations are float), or the
+// float state.
+__global__ void k_state64(const float *__restrict__ x, const float *__restrict__ y,
+                          const float *__restrict__ z, const float *__restrict__ vx,
+                          const float *__restrict__ vy, const float *__restrict__ vz,
+                          const float *__restrict__ ax, const float *__restrict__ ay,
+                          const float *__restrict__ az, const float *__restrict__ lo,
+                          const double *__restrict__ s64, int n, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *src[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const size_t q = (size_t)k * n + i;
+        out[q] = s64 ? s64[q] : (double)src[k][i] + (lo && k < 6 ? (double)lo[q] : 0.0);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_kick_drift_bbox_d(SegTable T, float *x, float *y,
+                                                              float *z, float *vx, float *vy,
+                                                              float *vz, double dt_half,
+                                                              double dt, double *s64,
+                                                              float *partial)
+{
+    const int b = blockIdx.x;
+    const int s = seg_of_block(T, b);
+    const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
+    const size_t n = T.n_total;
+    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
+    float hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+    if (i < T.off[s + 1]) {
+        float *xs[3] = {x, y, z};
+        float *vs[3] = {vx, vy, vz};
+        float o[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double V = s64[(3 + k) * n + i] + s64[(6 + k) * n + i] * dt_half;
+            const double X = s64[k * n + i] + V * dt;
+            s64[(3 + k) * n + i] = V;
+            s64[k * n + i] = X;
+            vs[k][i] = (float)V;
+            xs[k][i] = o[k] = (float)X;
+        }
+        lx = hx = o[0];
+        ly = hy = o[1];
+        lz = hz = o[2];
+    }
+    bbox_post(lx, ly, lz, hx, hy, hz, partial);
+}
+
+__global__ void k_kick_drag_d(float *vx, float *vy, float *vz, double dt_half, int has_drag,
+                              double decay, double star_mass, int n, int *step_ctr, double *s64)
+{
+    if (step_ctr && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step_ctr, 1);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t m = n;
+    double V[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) V[k] = s64[(3 + k) * m + i] + s64[(6 + k) * m + i] * dt_half;
+    if (has_drag) gas_drag_one_d(s64[i], s64[2 * m + i], V[0], V[1], V[2], decay, star_mass);
+    float *vs[3] = {vx, vy, vz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s64[(3 + k) * m + i] = V[k];
+        vs[k][i] = (float)V[k];
+    }
+}
+
 // FP64 diagnostics: block partial sums then one double atomic per block.
 __global__ __launch_bounds__(kBlock) void k_diag(const float *__restrict__ x,
                                                  const float *__restrict__ y,
@@ -245,7 +334,8 @@ __global__ __launch_bounds__(kBlock) void k_diag(const float *__restrict__ x,
                                                  const float *__restrict__ vy,
                                                  const float *__restrict__ vz,
                                                  const float *__restrict__ m,
-                                                 const float *__restrict__ lo, int n,
+                                                 const float *__restrict__ lo,
+                                                 const double *__restrict__ s64, int n,
                                                  double star_mass, double soft, double *out)
 {
     __shared__ double s[kBlock / kWave][6];
@@ -253,7 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_diag(const float *__restrict__ x,
     double e[6] = {0, 0, 0, 0, 0, 0};
     if (i < n) {
         double X = x[i], Y = y[i], Z = z[i], U = vx[i], V = vy[i], W = vz[i];
-        if (lo) {
+        if (s64) {
+            X = s64[i]; Y = s64[n + i]; Z = s64[2 * n + i];
+            U = s64[3 * n + i]; V = s64[4 * n + i]; W = s64[5 * n + i];
+        } else if (lo) {
             X += lo[i]; Y += lo[n + i]; Z += lo[2 * n + i];
             U += lo[3 * n + i]; V += lo[4 * n + i]; W += lo[5 * n + i];
         }
@@ -286,17 +379,23 @@ __global__ __launch_bounds__(kBlock) void k_diag_self(const float *__restrict__ 
                                                       const float *__restrict__ y,
                                                       const float *__restrict__ z,
                                                       const float *__restrict__ m,
-                                                      const float *__restrict__ lo, int n,
+                                                      const float *__restrict__ lo,
+                                                      const double *__restrict__ s64, int n,
                                                       double soft, double *out)
 {
     __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sm[kBlock];
     __shared__ double red[kBlock / kWave];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // position of body j: the FP64 state, hi + lo, or the float state
+    auto pos = [&](int j, int k) {
+        const float *f = k == 0 ? x : (k == 1 ? y : z);
+        return s64 ? s64[(size_t)k * n + j] : (double)f[j] + (lo ? (double)lo[k * n + j] : 0.0);
+    };
     double X = 0, Y = 0, Z = 0, M = 0;
     if (i < n) {
-        X = (double)x[i] + (lo ? lo[i] : 0.0);
-        Y = (double)y[i] + (lo ? lo[n + i] : 0.0);
-        Z = (double)z[i] + (lo ? lo[2 * n + i] : 0.0);
+        X = pos(i, 0);
+        Y = pos(i, 1);
+        Z = pos(i, 2);
         M = m[i];
     }
     const double e2 = soft * soft;
@@ -304,9 +403,9 @@ __global__ __launch_bounds__(kBlock) void k_diag_self(const float *__restrict__ 
     for (int base = 0; base < n; base += kBlock) {
         const int j = base + threadIdx.x;
         if (j < n) {
-            sx[threadIdx.x] = (double)x[j] + (lo ? lo[j] : 0.0);
-            sy[threadIdx.x] = (double)y[j] + (lo ? lo[n + j] : 0.0);
-            sz[threadIdx.x] = (double)z[j] + (lo ? lo[2 * n + j] : 0.0);
+            sx[threadIdx.x] = pos(j, 0);
+            sy[threadIdx.x] = pos(j, 1);
+            sz[threadIdx.x] = pos(j, 2);
             sm[threadIdx.x] = m[j];
         } else {
             sx[threadIdx.x] = sy[threadIdx.x] = sz[threadIdx.x] = 0;
@@ -1234,11 +1333,15 @@ __device__ inline DVec<N> dv_shfl_up(const DVec<N> &a, int o)
 // plain run of that disk alone.
 __host__ __device__ inline int pbase(const SegTable &T, int s) { return T.off[s] + s; }
 
+// KIND 0: {m, m x, m y, m z}; KIND 1: second moments; KIND 2: KIND 0 with
+// the positions of the FP64 state s64 ([x | y | z] in body order, read
+// through the sorted order perm) instead of the float bodies.
 template <int N, int KIND>
 __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__restrict__ bodies,
                                                    DVec<N> *P, DVec<N> *pagg,
                                                    DVec<N> *pinc, int *tickets, int slot,
-                                                   int *ghist)
+                                                   int *ghist, const double *__restrict__ s64,
+                                                   const int *__restrict__ perm)
 {
     __shared__ int s_tile;
     __shared__ DVec<N> s_w[kBlock / kWave];
@@ -1259,8 +1362,15 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
         DVec<N> e = dv_zero<N>();
         if (i < ns) {
             const float4 b = bodies[T.off[s] + i];
-            const double mm = b.w, x = b.x, y = b.y, z = b.z;
-            if (KIND == 0) {
+            const double mm = b.w;
+            double x = b.x, y = b.y, z = b.z;
+            if (KIND == 2) {
+                const int n = T.n_total, j = perm[T.off[s] + i];
+                x = s64[j];
+                y = s64[n + j];
+                z = s64[2 * n + j];
+            }
+            if (KIND != 1) {
                 e.v[0] = mm;
                 e.v[1] = mm * x;
                 e.v[2] = mm * y;
@@ -1338,7 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
         if (lane == 0) s_pre = pre;
         if (tile == first_tile && lane == 0) P[pbase(T, s)] = dv_zero<N>();
     }
-    if (KIND == 0 && tile == 0)  // histograms are consumed: clear them for the next build
+    if (KIND != 1 && tile == 0)  // histograms are consumed: clear them for the next build
         for (int q = t; q < kHistInts; q += kBlock) ghist[q] = 0;
     __syncthreads();
     DVec<N> run = dv_add(s_pre, dv_add(before, excl));
@@ -1793,14 +1903,45 @@ void launch_kick_drag_c(float *vx, float *vy, float *vz, const float *ax, const 
 }
 
 void launch_diagnostics(const float *x, const float *y, const float *z, const float *vx,
-                        const float *vy, const float *vz, const float *m, const float *lo, int n,
-                        double star_mass, double soft, int with_self, Bufs &B, hipStream_t st)
+                        const float *vy, const float *vz, const float *m, const float *lo,
+                        const double *s64, int n, double star_mass, double soft, int with_self,
+                        Bufs &B, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, B.diag, 0.0, 8);
-    hipLaunchKernelGGL(k_diag, NBX_GRID(n), x, y, z, vx, vy, vz, m, lo, n, star_mass, soft,
+    hipLaunchKernelGGL(k_diag, NBX_GRID(n), x, y, z, vx, vy, vz, m, lo, s64, n, star_mass, soft,
                        B.diag);
     if (with_self)
-        hipLaunchKernelGGL(k_diag_self, NBX_GRID(n), x, y, z, m, lo, n, soft, B.diag);
+        hipLaunchKernelGGL(k_diag_self, NBX_GRID(n), x, y, z, m, lo, s64, n, soft, B.diag);
+}
+
+void launch_load64(const float *x, const float *y, const float *z, const float *vx,
+                   const float *vy, const float *vz, double *s64, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_load64, NBX_GRID(n), x, y, z, vx, vy, vz, s64, n);
+}
+
+void launch_state64(const float *x, const float *y, const float *z, const float *vx,
+                    const float *vy, const float *vz, const float *ax, const float *ay,
+                    const float *az, const float *lo, const double *s64, int n, double *out,
+                    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_state64, NBX_GRID(n), x, y, z, vx, vy, vz, ax, ay, az, lo, s64, n, out);
+}
+
+void launch_kick_drift_bbox_d(const SegTable &T, float *x, float *y, float *z, float *vx,
+                              float *vy, float *vz, double dt_half, double dt, Bufs &B,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kick_drift_bbox_d, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
+                       vx, vy, vz, dt_half, dt, B.s64, B.partial);
+}
+
+void launch_kick_drag_d(float *vx, float *vy, float *vz, double dt_half, int has_drag,
+                        double decay, double star_mass, int n, int *step_ctr, Bufs &B,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_kick_drag_d, NBX_GRID(n), vx, vy, vz, dt_half, has_drag, decay,
+                       star_mass, n, step_ctr, B.s64);
 }
 
 void launch_bbox(const SegTable &T, const float *x, const float *y, const float *z, Bufs &B,
@@ -1879,7 +2020,8 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
 }
 
 void launch_build(const SegTable &T, const float *x, const float *y, const float *z,
-                  const float *mass, Bufs &B, hipStream_t st, bool resample, const LptDev &lpt)
+                  const float *mass, Bufs &B, hipStream_t st, bool resample, const LptDev &lpt,
+                  const double *s64)
 {
     const int n = T.n_total;
     if (B.radix_bits != 0) launch_lpt_scatter(lpt, st);
@@ -1889,12 +2031,18 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
     case 11: launch_sort<11>(T, x, y, z, mass, B, st); break;
     default: launch_sort<8>(T, x, y, z, mass, B, st); break;
     }
-    hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T, B.bodies,
-                       B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist);
+    if (s64)
+        hipLaunchKernelGGL((k_prefix<4, 2>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
+                           B.bodies, B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist,
+                           s64, B.vals[B.cur]);
+    else
+        hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
+                           B.bodies, B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist,
+                           nullptr, nullptr);
     if (B.quad)
         hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
                            B.bodies, B.prefix2, B.pagg2, B.pinc2, B.tickets,
-                           kTicketScan2, B.ghist);
+                           kTicketScan2, B.ghist, nullptr, nullptr);
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
         hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[B.cur], B.prefix,

@@ -26,7 +26,8 @@ _REPO = os.path.dirname(_PKG)
 CSRC = os.path.join(_PKG, "csrc")
 LIB_PATH = os.path.join(_HERE, "libnbx.so")
 RUN_PATH = os.path.join(_HERE, "nbx_run")
-HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_adjoint.hip", "nbx_abi.hip", "nbx_io.cpp"]
+HIP_SOURCES = ["nbx_tree.hip", "nbx_walk.hip", "nbx_fp64.hip", "nbx_adjoint.hip", "nbx_abi.hip",
+               "nbx_io.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep hand-written float code on single-lane VALU ops (no v_pk_*_f32)
@@ -113,7 +114,7 @@ EXPORTS = [
     "nbx_particle_radius", "nbx_init_disk", "nbx_writer_open", "nbx_writer_frame",
     "nbx_writer_stats", "nbx_writer_perturber", "nbx_writer_finalize", "nbx_writer_close",
     "nbx_write_perturber_sidecar", "nbx_fof_noalloc", "nbx_fof_disks", "nbx_search_stats",
-    "nbx_tree_set_wave_order",
+    "nbx_tree_set_wave_order", "nbx_state64",
 ]
 
 _lib = None
@@ -157,6 +158,7 @@ def lib():
             "nbx_walk_counts": [vp, f32, vp, vp],
             "nbx_tree_set_multipole": [vp, i32],
             "nbx_tree_set_precision": [vp, i32],
+            "nbx_state64": [vp, P, vp],
             "nbx_diagnostics": [vp, P, f32, f32, i32, C.POINTER(C.c_double)],
             "nbx_tree_set_walk_shape": [vp, i32],
             "nbx_tree_get_walk_shape": [vp, C.POINTER(i32)],
@@ -660,9 +662,25 @@ def set_multipole(t, order):
     _check(lib().nbx_tree_set_multipole(t.h, int(order)), "set_multipole")
 
 
-def set_precision(t, compensated):
-    """True: FP64-compensated KDK updates (forces stay FP32)."""
-    _check(lib().nbx_tree_set_precision(t.h, int(bool(compensated))), "set_precision")
+PRECISION_FLOAT, PRECISION_COMPENSATED, PRECISION_FP64 = 0, 1, 2
+_PRECISIONS = {"float": 0, "compensated": 1, "fp64": 2}
+
+
+def set_precision(t, mode):
+    """Precision of nbx_step: "float" (or False / 0), "compensated" (or True
+    / 1: FP64-compensated KDK, forces FP32), "fp64" (2: the all-FP64 step,
+    loaded from the float state by the next init_forces)."""
+    m = _PRECISIONS[mode] if isinstance(mode, str) else int(mode)
+    _check(lib().nbx_tree_set_precision(t.h, m), "set_precision")
+
+
+def state64(t, p):
+    """The state as a (9, n) float64 CUDA tensor [x, y, z, vx, vy, vz, ax,
+    ay, az]: the FP64 state, hi + lo (compensated), or the float state."""
+    import torch
+    out = torch.empty((9, p.n), dtype=torch.float64, device="cuda")
+    _check(lib().nbx_state64(t.h, C.byref(p.c()), C.c_void_p(out.data_ptr())), "state64")
+    return out
 
 
 def diagnostics(t, p, star_mass=1.0, softening=1e-3, with_self=False):
