@@ -161,7 +161,7 @@ int nbx_fof_disks(nbx_tree *t, const nbx_particles *p, const float *link_lengths
 /* No counterpart in the reference: nbx_fof for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that
  * forbid hipMalloc).  Same labels as nbx_fof, with no tree: the scratch is a
- * static device array of 20 MiB per GPU (allocated when the library is
+ * static device array of 40 MiB per GPU (allocated when the library is
  * loaded, held until the process ends), so at most NBX_NOALLOC_MAX_BODIES
  * bodies (larger n: NBX_E_ARG before any device work).  No hipMalloc,
  * hipFree or other allocating call.  Work goes on `stream` (a hipStream_t;

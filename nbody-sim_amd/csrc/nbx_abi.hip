@@ -295,7 +295,8 @@ int make_segtable(int nseg, const int32_t *counts, SegTable &T)
         T.off[s] = (int)total;
         total += counts[s];
     }
-    if (total > (1L << 28)) return fail(NBX_E_ARG, "too many bodies (%ld)", total);
+    // walk-record indices (8 n + 8 per disk) stay below 2^31
+    if (total > (1L << 27)) return fail(NBX_E_ARG, "too many bodies (%ld > 2^27)", total);
     T.off[nseg] = (int)total;
     T.n_total = (int)total;
     for (int s = 0; s < nseg; ++s) {
@@ -309,7 +310,7 @@ int make_segtable(int nseg, const int32_t *counts, SegTable &T)
         T.blk[s + 1] = T.blk[s] + (n + kBlock - 1) / kBlock;
         T.wave[s + 1] = T.wave[s] + (n + kWave - 1) / kWave;
         T.ioff[s + 1] = T.ioff[s] + (n - 1);
-        T.noff[s + 1] = T.noff[s] + 2 * n + 1;
+        T.noff[s + 1] = T.noff[s] + 8 * n + 8;  // root slot + an 8-record block per node
     }
     for (int s = nseg + 1; s <= kMaxSeg; ++s) {
         T.off[s] = T.off[nseg];
@@ -559,7 +560,7 @@ int nbx_tree_set_precision(nbx_tree *t, int mode)
     if (mode == NBX_PRECISION_FP64) {
         // stack entries of the FP64 walk pack a record index above 4 bits
         if (t->T.noff[t->T.nseg] >= (1 << 27))
-            return fail(NBX_E_ARG, "the FP64 step takes at most 2^27 walk records");
+            return fail(NBX_E_ARG, "the FP64 step takes at most 2^24 bodies");
         int rc;
         if ((!B.s64 && (rc = dmalloc(&B.s64, (size_t)9 * n))) ||
             (!B.rec64 && (rc = dmalloc(&B.rec64, (size_t)B.n_rec))) ||
@@ -1413,11 +1414,13 @@ int nbx_tree_stats(nbx_tree *t, int32_t *n_nodes, int32_t *max_level)
     if (!t) return fail(NBX_E_ARG, "null tree");
     if (t->stage < kSummarized) return fail(NBX_E_STATE, "statistics before summarize");
     std::vector<int> alloc(kMaxSeg), lev(kMaxSeg);
+    launch_tree_stats(t->T, t->B, t->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(t->stream));
     HIPCHK(hipMemcpy(alloc.data(), t->B.alloc, sizeof(int) * kMaxSeg, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(lev.data(), t->B.maxlev, sizeof(int) * kMaxSeg, hipMemcpyDeviceToHost));
     for (int s = 0; s < t->T.nseg; ++s) {
-        if (n_nodes) n_nodes[s] = alloc[s] + 1;  // records incl. the root
+        if (n_nodes) n_nodes[s] = alloc[s] + 1;  // child records + the root
         if (max_level) max_level[s] = lev[s];
     }
     return NBX_OK;
@@ -1462,10 +1465,10 @@ int nbx_phase_ms(nbx_tree *t, const char *phase, float *ms)
 namespace {
 
 // Bytes of the static scratch: the core layout of one disk of
-// NBX_NOALLOC_MAX_BODIES bodies (about 284 bytes per body plus the fixed
-// histogram and padding) fits with margin; checked when a device first
-// uses it.
-constexpr size_t kNoAllocScratchBytes = (size_t)20 << 20;
+// NBX_NOALLOC_MAX_BODIES bodies (about 570 bytes per body, 384 of them the
+// walk records' 8-record blocks, plus the fixed histogram and padding) fits
+// with margin; checked when a device first uses it.
+constexpr size_t kNoAllocScratchBytes = (size_t)40 << 20;
 constexpr int kMaxDevices = 64;
 
 }  // namespace
@@ -1547,7 +1550,7 @@ extern "C" int nbx_fof_noalloc(const nbx_particles *p, float link_length, int32_
         return fail(NBX_E_STATE, "nbx_fof_noalloc: scratch layout overflow (internal)");
     hipStream_t st = static_cast<hipStream_t>(stream);
     // the previous call's contents stay in the scratch: the error flags
-    // start from zero (the key pass resets its own tickets and allocators)
+    // start from zero (the key pass resets its own tickets)
     launch_fill_int(B.err, 0, 4, st);
     launch_bbox(T, p->x, p->y, p->z, B, st);
     launch_root(T, B, st);

@@ -21,30 +21,22 @@ namespace nbx {
 constexpr int kStack64 = kLevels + 3;  // root + one block per level + a level-21 bucket
 constexpr int kDirect64 = 64;          // nodes of at most this many bodies sum their members
 
-// FP64 {com, mass} of every used walk record.  A leaf is its body; a node of
-// at most kDirect64 bodies sums its members directly; a larger node takes the
-// difference of the FP64 prefix of {m, m x, m y, m z} summed from the FP64
-// positions (its cancellation error is at most ~1e-16 of the disk's moment
-// over the node's mass, below 1e-12 AU here).
-__global__ __launch_bounds__(kBlock) void k_rec64(SegTable T, const int4 *__restrict__ rb,
-                                                  const int2 *__restrict__ brange,
-                                                  const DVec<4> *__restrict__ P,
-                                                  const double *__restrict__ s64,
-                                                  const int *__restrict__ perm,
-                                                  const float4 *__restrict__ bodies,
-                                                  const int *__restrict__ alloc, double4 *rec64)
+// FP64 {com, mass} of every walk record, indexed like the record pass: one
+// thread per (internal node, child slot) plus the root of each disk.  A leaf
+// is its body; a node of at most kDirect64 bodies sums its members directly;
+// a larger node takes the difference of the FP64 prefix of {m, m x, m y, m z}
+// summed from the FP64 positions (its cancellation error is at most ~1e-16 of
+// the disk's moment over the node's mass, below 1e-12 AU here).
+__device__ inline double4 moments64(const SegTable &T, int s, int code,
+                                    const int2 *__restrict__ brange,
+                                    const DVec<4> *__restrict__ P, const double *__restrict__ s64,
+                                    const int *__restrict__ perm,
+                                    const float4 *__restrict__ bodies)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= T.noff[T.nseg]) return;
-    int s = 0;
-    while (s + 1 < T.nseg && r >= T.noff[s + 1]) ++s;
-    if (r - T.noff[s] > alloc[s]) return;  // disk s uses [noff, noff + 1 + alloc)
     const size_t n = T.n_total;
-    const int code = rb[r].x;
     if (code < 0) {  // leaf: the body itself
         const int g = ~code, i = perm[g];
-        rec64[r] = make_double4(s64[i], s64[n + i], s64[2 * n + i], (double)bodies[g].w);
-        return;
+        return make_double4(s64[i], s64[n + i], s64[2 * n + i], (double)bodies[g].w);
     }
     const int2 q = brange[code];  // sorted positions [lo, hi] of the node
     double m = 0, cx = 0, cy = 0, cz = 0;
@@ -65,13 +57,32 @@ __global__ __launch_bounds__(kBlock) void k_rec64(SegTable T, const int4 *__rest
         cy = b.v[2] - a.v[2];
         cz = b.v[3] - a.v[3];
     }
-    if (m > 0.0) {
-        rec64[r] = make_double4(cx / m, cy / m, cz / m, m);
-    } else {  // massless node: the midpoint of its end bodies (it pulls nothing)
-        const int i = perm[q.x], j = perm[q.y];
-        rec64[r] = make_double4(0.5 * (s64[i] + s64[j]), 0.5 * (s64[n + i] + s64[n + j]),
-                                0.5 * (s64[2 * n + i] + s64[2 * n + j]), 0.0);
+    if (m > 0.0) return make_double4(cx / m, cy / m, cz / m, m);
+    // massless node: the midpoint of its end bodies (it pulls nothing)
+    const int i = perm[q.x], j = perm[q.y];
+    return make_double4(0.5 * (s64[i] + s64[j]), 0.5 * (s64[n + i] + s64[n + j]),
+                        0.5 * (s64[2 * n + i] + s64[2 * n + j]), 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rec64(SegTable T, const int *__restrict__ cnt,
+                                                  const int *__restrict__ first,
+                                                  const int *__restrict__ kids,
+                                                  const int2 *__restrict__ brange,
+                                                  const DVec<4> *__restrict__ P,
+                                                  const double *__restrict__ s64,
+                                                  const int *__restrict__ perm,
+                                                  const float4 *__restrict__ bodies,
+                                                  double4 *rec64)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = tid >> 3, k = tid & 7;
+    if (k == 0 && j < T.nseg) {  // root of disk j: its node 0, or its one body
+        const int code = T.off[j + 1] - T.off[j] == 1 ? ~T.off[j] : T.ioff[j];
+        rec64[T.noff[j]] = moments64(T, j, code, brange, P, s64, perm, bodies);
     }
+    if (j >= T.ioff[T.nseg] || k >= cnt[j]) return;
+    const int s = seg_of_internal(T, j);
+    rec64[first[j] + k] = moments64(T, s, kids[8 * j + k], brange, P, s64, perm, bodies);
 }
 
 __global__ __launch_bounds__(kBlock) void k_walk64(SegTable T, const double4 *__restrict__ rec64,
@@ -157,9 +168,10 @@ static inline int cdiv64(int a, int b) { return (a + b - 1) / b; }
 
 void launch_rec64(const SegTable &T, Bufs &B, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_rec64, dim3(cdiv64(T.noff[T.nseg], kBlock)), dim3(kBlock), 0, st, T,
-                       B.rec_b, B.brange, B.prefix, B.s64, B.vals[B.cur], B.bodies, B.alloc,
-                       B.rec64);
+    const int ni = T.ioff[T.nseg];
+    hipLaunchKernelGGL(k_rec64, dim3(cdiv64(8 * (ni > kMaxSeg ? ni : kMaxSeg), kBlock)),
+                       dim3(kBlock), 0, st, T, B.cnt, B.first, B.kids, B.brange, B.prefix, B.s64,
+                       B.vals[B.cur], B.bodies, B.rec64);
 }
 
 void launch_walk64(const SegTable &T, const DiskDev *dtab, const int *step_ctr, double theta2,

@@ -10,8 +10,9 @@
 //   rec_a/b    walk records of the compressed octree: float4 {com, mass}
 //              and int4 {code, h^2, first child record, child count} (code:
 //              the binary node of the record, ~sorted position for a leaf); the
-//              children of a node are contiguous, the root record of disk s
-//              sits at node_off[s].
+//              children of internal node j fill the head of its own 8-record
+//              block at noff[s] + 8 + 8 (j - ioff[s]); the root record of
+//              disk s sits at noff[s].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,7 +47,7 @@ struct SegTable {
     int blk[kMaxSeg + 1];        // kBlock-sized block offset of disk s
     int wave[kMaxSeg + 1];       // wave (64 targets) offset of disk s
     int ioff[kMaxSeg + 1];       // binary internal-node offset (n_s - 1 each)
-    int noff[kMaxSeg + 1];       // walk-record offset (2 n_s + 1 each)
+    int noff[kMaxSeg + 1];       // walk-record offset (8 n_s + 8 each: root, 8 per node)
     int stile[kMaxSeg + 1];      // prefix-scan tile offset (tiles never straddle disks)
     int bkoff[kMaxSeg + 1];      // sort-bucket offset (kBucketMean bodies per bucket)
     int sblk[kMaxSeg + 1];       // scatter-block offset (kSplitTile bodies per block)

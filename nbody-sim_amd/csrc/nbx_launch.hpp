@@ -50,7 +50,7 @@ struct Bufs {
     int *oct = nullptr;          // [n] 1 if the binary node is an octree node
     int *cnt = nullptr;          // [n] octree child count (0: not an octree node)
     int *first = nullptr;        // [n] first child record of octree node
-    int *alloc = nullptr;        // [kMaxSeg] record allocators per disk
+    int *alloc = nullptr;        // [kMaxSeg] child records per disk (statistics)
     int *maxlev = nullptr;       // [kMaxSeg] deepest octree level per disk
     float4 *bin_com = nullptr;   // [n] {com, mass} of binary nodes
     float4 *bin_cell = nullptr;  // [n] {cell centre, half size} of binary nodes
@@ -177,6 +177,9 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
                   const float *mass, Bufs &B, hipStream_t st, bool resample = true,
                   const LptDev &lpt = LptDev{}, const double *s64 = nullptr);
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st);
+// Child records per disk (the root not included) into B.alloc, deepest
+// octree level into B.maxlev (statistics only).
+void launch_tree_stats(const SegTable &T, Bufs &B, hipStream_t st);
 void launch_fill_int(int *p, int v, int n, hipStream_t st);
 void launch_fill_float(float *p, float v, int n, hipStream_t st);
 

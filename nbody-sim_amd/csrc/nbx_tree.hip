@@ -6,7 +6,7 @@
 //   quantiles) -> bucket sort in LDS (sorted keys, indices and bodies) ->
 //   FP64 prefix scan of {m, m x, m y, m z} -> binary radix tree (one thread
 //   per internal node) with node mass / COM from the prefix, octree flags
-//   and cell geometry -> record allocation -> records.  Disks above 2^20
+//   and cell geometry -> child counts -> records.  Disks above 2^20
 //   bodies (or a set digit width) sort with LSD radix passes instead.
 // Every phase is HBM-streaming integer/float work; nothing here is a GEMM.
 #include <float.h>
@@ -560,7 +560,7 @@ __device__ void keys_prologue(const SegTable &T, const float *__restrict__ parti
                               Root *__restrict__ roots, Root *s_root, int s0, int s1, int nblocks,
                               unsigned *__restrict__ status0, int n_status,
                               unsigned long long *__restrict__ scanw, int n_scanw,
-                              int *__restrict__ tickets, int *__restrict__ alloc)
+                              int *__restrict__ tickets)
 {
     if (blockIdx.x == 0) {
         s0 = 0;
@@ -572,7 +572,6 @@ __device__ void keys_prologue(const SegTable &T, const float *__restrict__ parti
     for (int q = gid; q < n_status; q += gsz) status0[q] = 0;
     for (int q = gid; q < n_scanw; q += gsz) scanw[q] = kScanEmpty;
     if (gid <= kTicketScan2) tickets[gid] = 0;  // tickets of the radix passes and the scans
-    if (gid < 2 * kMaxSeg) alloc[gid] = 0;      // record allocators and depths of the records pass
 }
 
 template <int BITS>
@@ -585,8 +584,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
                                                  int *__restrict__ vals, int *__restrict__ ghist,
                                                  unsigned *__restrict__ status0, int n_status,
                                                  unsigned long long *__restrict__ scanw,
-                                                 int n_scanw, int *__restrict__ tickets,
-                                                 int *__restrict__ alloc)
+                                                 int n_scanw, int *__restrict__ tickets)
 {
     constexpr int R = 1 << BITS, NP = (63 + BITS - 1) / BITS;
     __shared__ int h[NP * R];
@@ -598,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(SegTable T, const float *__rest
         const int b0 = blockIdx.x * kKeysPerBlock;
         const int b1 = min(n, b0 + kKeysPerBlock) - 1;
         keys_prologue(T, partial, roots, s_root, seg_of_body(T, b0), seg_of_body(T, b1),
-                      gridDim.x, status0, n_status, scanw, n_scanw, tickets, alloc);
+                      gridDim.x, status0, n_status, scanw, n_scanw, tickets);
     }
     __syncthreads();
     float px[kKeysItems], py[kKeysItems], pz[kKeysItems];
@@ -1117,7 +1115,7 @@ __global__ __launch_bounds__(kBlock) void k_keys_scatter(
     SegTable T, const float *__restrict__ x, const float *__restrict__ y,
     const float *__restrict__ z, const float *__restrict__ m, const float *__restrict__ partial,
     Root *__restrict__ roots, unsigned long long *__restrict__ scanw, int n_scanw,
-    int *__restrict__ tickets, int *__restrict__ alloc, const uint64_t *__restrict__ spk,
+    int *__restrict__ tickets, const uint64_t *__restrict__ spk,
     const int *__restrict__ spi, uint64_t *__restrict__ skey, int *__restrict__ sidx,
     float4 *__restrict__ spay, int *__restrict__ bm, LptDev L)
 {
@@ -1140,7 +1138,7 @@ __global__ __launch_bounds__(kBlock) void k_keys_scatter(
         pm[r] = i < end ? m[i] : 0.f;
     }
     keys_prologue(T, partial, roots, s_root, s, s, T.sblk[T.nseg], nullptr, 0, scanw, n_scanw,
-                  tickets, alloc);
+                  tickets);
     __syncthreads();
     uint64_t kk[kSplitItems];
     float4 pay[kSplitItems];
@@ -1695,50 +1693,58 @@ __device__ inline int oct_children(const int *__restrict__ left, const int *__re
     return -1;
 }
 
-// Child counts and record blocks: one ticketed atomic per block and disk.
-// The children found (at most 8 codes per octree node) are kept in kids[8 j ..]
-// for the record pass.
+// Child counts.  Every internal node owns an 8-record block of the disk's
+// records (first[j] = noff[s] + 8 + 8 (j - ioff[s]); the root record sits at
+// noff[s]), so no allocation crosses threads: an octree node's children fill
+// the head of its block, 8-aligned (one 128-byte line per record array), and
+// the blocks of folded nodes stay unused.  The children found (at most 8
+// codes per octree node) are kept in kids[8 j ..] for the record pass.
 __global__ __launch_bounds__(kBlock) void k_oct_count(SegTable T, const int *__restrict__ left,
                                                       const int *__restrict__ right,
-                                                      const int *__restrict__ octf,
-                                                      const int *__restrict__ plen, int *cnt,
-                                                      int *first, int *alloc, int *maxlev,
-                                                      int *err, int *__restrict__ kids)
+                                                      const int *__restrict__ octf, int *cnt,
+                                                      int *first, int *err,
+                                                      int *__restrict__ kids)
 {
-    __shared__ int s_base[kMaxSeg];
-    __shared__ int s_tot[kMaxSeg];
-    __shared__ int s_lev[kMaxSeg];
     const int ni = T.ioff[T.nseg];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ni) return;
+    int c = 0;
+    if (octf[j]) {
+        c = oct_children(left, right, octf, j, kids + 8 * j);
+        if (c < 0 || c > 8) {
+            atomicOr(err, kErrTree);
+            c = 0;
+        }
+    }
+    cnt[j] = c;
+    if (c) {
+        const int s = seg_of_internal(T, j);
+        first[j] = T.noff[s] + 8 + 8 * (j - T.ioff[s]);
+    }
+}
+
+// Record count (root included) and deepest octree level of every disk, for
+// the statistics call: block sums in LDS, one atomic per block and disk.
+__global__ __launch_bounds__(kBlock) void k_tree_stats(SegTable T, const int *__restrict__ cnt,
+                                                       const int *__restrict__ plen, int *nodes,
+                                                       int *maxlev)
+{
+    __shared__ int s_tot[kMaxSeg], s_lev[kMaxSeg];
     if (threadIdx.x < kMaxSeg) {
         s_tot[threadIdx.x] = 0;
         s_lev[threadIdx.x] = 0;
     }
     __syncthreads();
-    int c = 0, s = 0, off = 0;
-    if (j < ni) {
-        s = seg_of_internal(T, j);
-        if (octf[j]) {
-            c = oct_children(left, right, octf, j, kids + 8 * j);
-            if (c < 0 || c > 8) {
-                atomicOr(err, kErrTree);
-                c = 0;
-            }
-        }
-        if (c) {
-            off = atomicAdd(&s_tot[s], c);  // LDS: order inside the block is irrelevant
-            atomicMax(&s_lev[s], oct_level(plen[j]));
-        }
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < T.ioff[T.nseg] && cnt[j] > 0) {
+        const int s = seg_of_internal(T, j);
+        atomicAdd(&s_tot[s], cnt[j]);
+        atomicMax(&s_lev[s], oct_level(plen[j]));
     }
     __syncthreads();
-    if (threadIdx.x < kMaxSeg && s_tot[threadIdx.x]) {
-        s_base[threadIdx.x] = atomicAdd(&alloc[threadIdx.x], s_tot[threadIdx.x]);
-        atomicMax(&maxlev[threadIdx.x], s_lev[threadIdx.x]);
-    }
-    __syncthreads();
-    if (j < ni) {
-        cnt[j] = c;
-        if (c) first[j] = T.noff[s] + 1 + s_base[s] + off;
+    if (threadIdx.x < T.nseg) {
+        if (s_tot[threadIdx.x]) atomicAdd(&nodes[threadIdx.x], s_tot[threadIdx.x]);
+        if (s_lev[threadIdx.x]) atomicMax(&maxlev[threadIdx.x], s_lev[threadIdx.x]);
     }
 }
 
@@ -1962,7 +1968,7 @@ static void launch_sort(const SegTable &T, const float *x, const float *y, const
     hipLaunchKernelGGL(k_keys<BITS>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
                        z, B.partial, B.roots, B.keys[0], B.vals[0], B.ghist, B.status[0],
                        R * B.n_sweep_tiles, (unsigned long long *)B.scanw,
-                       kScanWords * B.n_scan_tiles, B.tickets, B.alloc);
+                       kScanWords * B.n_scan_tiles, B.tickets);
     const int npass = NP + (T.nseg > 1 ? 1 : 0);
     int cur = 0;
     for (int p = 0; p < npass; ++p) {
@@ -1998,8 +2004,8 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
         launch_lpt_scatter(lpt, st);
         hipLaunchKernelGGL(k_keys<8>, dim3(cdiv(n, kKeysPerBlock)), dim3(kBlock), 0, st, T, x, y,
                            z, B.partial, B.roots, B.keys[0], nullptr, nullptr, nullptr, 0,
-                           (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
-                           B.alloc);
+                           (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles,
+                           B.tickets);
         hipLaunchKernelGGL(k_split, dim3(T.nseg), dim3(kSplitThreads), 0, st, T, B.keys[0],
                            B.spk, B.spi);
         hipLaunchKernelGGL(k_scatter, dim3(T.sblk[T.nseg]), dim3(kBlock), 0, st, T, B.keys[0], x,
@@ -2008,7 +2014,7 @@ static void launch_split_sort(const SegTable &T, const float *x, const float *y,
         hipLaunchKernelGGL(k_keys_scatter, dim3(T.sblk[T.nseg] + lpt_parts(lpt.nw)),
                            dim3(kBlock), 0, st, T, x, y, z, mass, B.partial, B.roots,
                            (unsigned long long *)B.scanw, kScanWords * B.n_scan_tiles, B.tickets,
-                           B.alloc, B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm, lpt);
+                           B.spk, B.spi, B.keys[1], B.vals[1], B.spay, B.bm, lpt);
     }
     int nblk_max = 1;
     for (int q = 0; q < T.nseg; ++q) nblk_max = std::max(nblk_max, T.sblk[q + 1] - T.sblk[q]);
@@ -2050,12 +2056,20 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
                            B.plen, B.oct, B.bin_com, B.bin_cell, B.bin_q, B.brange);
 }
 
+void launch_tree_stats(const SegTable &T, Bufs &B, hipStream_t st)
+{
+    launch_fill_int(B.alloc, 0, 2 * kMaxSeg, st);
+    const int ni = T.ioff[T.nseg];
+    if (ni > 0)
+        hipLaunchKernelGGL(k_tree_stats, NBX_GRID(ni), T, B.cnt, B.plen, B.alloc, B.maxlev);
+}
+
 void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
 {
     const int ni = T.ioff[T.nseg];
     if (ni > 0)
-        hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.plen, B.cnt,
-                           B.first, B.alloc, B.maxlev, B.err, B.kids);
+        hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.cnt, B.first,
+                           B.err, B.kids);
     hipLaunchKernelGGL(k_oct_fill, NBX_GRID(8 * (ni > kMaxSeg ? ni : kMaxSeg)), T, B.cnt, B.first,
                        B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a, B.rec_b, B.rec_c,
                        B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar, B.opar_leaf, B.kids);
