@@ -317,7 +317,6 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (nrec_n == 0 && sp > 0) fetch();  // the stack was empty when prefetching
     }
-    if (lane == 0) atomicMax(err + 2, sp_max);  // deepest stack (diagnostic)
     if (wstat && lane == 0) {  // per-wave work: records, batches, deepest stack
         wstat[0] = n_pop;
         wstat[1] = n_batch;
