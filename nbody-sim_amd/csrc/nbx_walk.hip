@@ -209,6 +209,15 @@ __device__ inline int search_start(const int *__restrict__ opar, const int *__re
     }
 }
 
+// A record as the per-lane searches read it: the cell {centre | body, h}
+// and {first child | sorted position, child count} (rec_b.z / .w, one 64-bit
+// load); the record's binary node (rec_b.x) is read only where a search
+// needs it, so a chunk of 8 records costs 48 registers.
+__device__ inline int2 rec_link(const int4 *__restrict__ rb, int r)
+{
+    return reinterpret_cast<const int2 *>(rb)[2 * r + 1];
+}
+
 // Depth-first search, in pre-order with children in block (Morton) order,
 // of the records below octree node `top` (binary index), without a stack:
 // the lane keeps the node whose child block it scans and the slot in that
@@ -244,31 +253,33 @@ __device__ inline void lane_walk_subtree(const float4 *__restrict__ rc,
             k = at + 1;
             continue;
         }
-        int4 B[8];
+        int2 L[8];
         float4 Cc[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            B[q] = rb[f + k + q];  // records past the block exist (padding / later blocks)
+            L[q] = rec_link(rb, f + k + q);  // records past the block exist (padding / later blocks)
             Cc[q] = rc[f + k + q];
         }
         const int kend = min(c, k + 8);
         int down = -1;
+        int2 d = make_int2(0, 0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if (k + q >= kend || down >= 0) break;
-            if (B[q].w == 0) {
-                pair(Cc[q], B[q].z);
+            if (L[q].y == 0) {
+                pair(Cc[q], L[q].x);
             } else if (near_cell(tx, ty, tz, Cc[q], radius, guard)) {
                 down = q;
+                d = L[q];
             }
         }
         if (down < 0) {
             k = kend;
             continue;
         }
-        node = B[down].x;  // the record's own binary node
-        f = B[down].z;
-        c = B[down].w;
+        node = rb[f + k + down].x;  // the record's own binary node
+        f = d.x;
+        c = d.y;
         k = 0;
     }
 }
@@ -277,8 +288,9 @@ __device__ inline void lane_walk_subtree(const float4 *__restrict__ rc,
 // count} on a per-lane stack of CAP blocks in LDS: a block is scanned
 // whole (one round trip per 8 records) and the near internal records among
 // its records are pushed.  A lane whose stack is full (a clump with every
-// cell in range: up to 7 pending blocks per level) searches that record's
-// subtree at once without a stack, so no depth or clump is out of reach.
+// cell in range: up to 7 pending blocks per level) searches those records'
+// subtrees without a stack once the chunk is scanned, so no depth or clump
+// is out of reach.
 template <int CAP, class Pair>
 __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4 *__restrict__ rb,
                                        const SearchTree &S, int first0, int count0, float tx,
@@ -293,27 +305,36 @@ __device__ inline void lane_walk_block(const float4 *__restrict__ rc, const int4
         const int2 e = st[sp * kWave];
         const int first = e.x, count = e.y;
         for (int k0 = 0; k0 < count; k0 += 8) {
-            int4 B[8];
-            float4 Cc[8];
+            unsigned over = 0;  // near internal records that found the stack full
+            {
+                int2 L[8];
+                float4 Cc[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                B[k] = rb[first + k0 + k];
-                Cc[k] = rc[first + k0 + k];
-            }
+                for (int k = 0; k < 8; ++k) {
+                    L[k] = rec_link(rb, first + k0 + k);
+                    Cc[k] = rc[first + k0 + k];
+                }
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k0 + k >= count) break;
-                if (B[k].w == 0) {
-                    pair(Cc[k], B[k].z);
-                } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
-                    if (sp < CAP) {
-                        st[sp * kWave] = make_int2(B[k].z, B[k].w);
-                        ++sp;
-                    } else {
-                        atomicAdd(err + 3, 1);  // diagnostic: searches past the stack
-                        lane_walk_subtree(rc, rb, S, B[k].x, tx, ty, tz, radius, guard, pair);
+                for (int k = 0; k < 8; ++k) {
+                    if (k0 + k >= count) break;
+                    if (L[k].y == 0) {
+                        pair(Cc[k], L[k].x);
+                    } else if (near_cell(tx, ty, tz, Cc[k], radius, guard)) {
+                        if (sp < CAP) {
+                            st[sp * kWave] = L[k];
+                            ++sp;
+                        } else {
+                            over |= 1u << k;
+                        }
                     }
                 }
+            }
+            while (over) {
+                const int k = __builtin_ctz(over);
+                over &= over - 1;
+                atomicAdd(err + 3, 1);  // diagnostic: searches past the stack
+                lane_walk_subtree(rc, rb, S, rb[first + k0 + k].x, tx, ty, tz, radius, guard,
+                                  pair);
             }
         }
     }
@@ -397,7 +418,7 @@ __device__ inline void collide_wave(const SegTable &T, const CollideArgs &C,
     if (v.pairs) atomicAdd(err + 1, v.pairs);
 }
 
-__global__ __launch_bounds__(kWalkBlock) void k_collide(SegTable T, CollideArgs C,
+__global__ __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_collide(SegTable T, CollideArgs C,
                                                         const int4 *__restrict__ rb,
                                                         const float4 *__restrict__ bodies,
                                                         const int *__restrict__ perm, int *err,
@@ -417,9 +438,11 @@ __global__ __launch_bounds__(kWalkBlock) void k_collide(SegTable T, CollideArgs 
 // come first, so they start first; most of their waves end within
 // microseconds and the walk fills the compute units; no second stream and
 // no cross-stream hand-off before or after the walk.
+// Five waves per SIMD, as the walk alone gets from its LDS: the collision
+// path is held to the register budget that allows it.
 template <int TG, bool QUAD, int U, bool SELF>
-__global__ __launch_bounds__(kGravBlock) void k_walk_collide(SegTable T, WaveMap M, GravArgs A,
-                                                             int ncb, CollideArgs C, SearchTree S)
+__global__ __launch_bounds__(kGravBlock) __attribute__((amdgpu_waves_per_eu(5))) void
+k_walk_collide(SegTable T, WaveMap M, GravArgs A, int ncb, CollideArgs C, SearchTree S)
 {
     __shared__ GravLds<TG, QUAD> lds[kGravWaves];
     static_assert(sizeof(lds) >= sizeof(int2) * kLStackC * kGravBlock,
