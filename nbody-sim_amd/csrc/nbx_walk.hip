@@ -147,6 +147,9 @@ constexpr int kLStack = 32;   // per-lane stack entries of the FoF searches (LDS
 // launch (search radii are tiny: a stack of 12 blocks holds every search
 // but those below dense clumps, which go on without it).
 constexpr int kLStackC = 12;
+// Body waves per collision wave in the fused walk launch: fewer collision
+// blocks leave the compute units' slots to the walk's blocks.
+constexpr int kCollideRounds = 2;
 
 // Tree arrays of the per-lane searches: octree parents (binary index) of
 // octree nodes and of body leaves, every octree node's child block and the
@@ -447,12 +450,11 @@ k_walk_collide(SegTable T, WaveMap M, GravArgs A, int ncb, CollideArgs C, Search
     __shared__ GravLds<TG, QUAD> lds[kGravWaves];
     static_assert(sizeof(lds) >= sizeof(int2) * kLStackC * kGravBlock,
                   "the collision stacks fit in the walk's LDS");
-    if ((int)blockIdx.x < ncb) {
+    if ((int)blockIdx.x < ncb) {  // each collision wave takes every (ncb * 4)-th body wave
         const int wib = threadIdx.x >> 6;
-        const int w = blockIdx.x * kGravWaves + wib;
-        if (w >= T.wave[T.nseg]) return;
         int2 *st = reinterpret_cast<int2 *>(lds) + wib * kLStackC * kWave + __lane_id();
-        collide_wave<kLStackC>(T, C, A.rb, A.bodies, A.perm, A.err, S, w, st);
+        for (int w = blockIdx.x * kGravWaves + wib; w < T.wave[T.nseg]; w += ncb * kGravWaves)
+            collide_wave<kLStackC>(T, C, A.rb, A.bodies, A.perm, A.err, S, w, st);
         return;
     }
     walk_block<TG, QUAD, U, SELF>(T, M, A, blockIdx.x - ncb, lds);
@@ -675,7 +677,7 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     }
     // collision blocks: a multiple of the XCD count, so the walk blocks keep
     // their die assignment
-    const int ncb = kXcds * cdiv(cdiv(T.wave[T.nseg], kGravWaves), kXcds);
+    const int ncb = kXcds * cdiv(cdiv(T.wave[T.nseg], kGravWaves * kCollideRounds), kXcds);
     auto kern = !self ? (W.quad ? k_walk_collide<TG, true, U, false>
                                 : k_walk_collide<TG, false, U, false>)
                       : (W.quad ? k_walk_collide<TG, true, U, true>
