@@ -1362,13 +1362,13 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
             const float4 b = bodies[T.off[s] + i];
             const double mm = b.w;
             double x = b.x, y = b.y, z = b.z;
-            if (KIND == 2) {
+            if constexpr (KIND == 2) {
                 const int n = T.n_total, j = perm[T.off[s] + i];
                 x = s64[j];
                 y = s64[n + j];
                 z = s64[2 * n + j];
             }
-            if (KIND != 1) {
+            if constexpr (KIND != 1) {
                 e.v[0] = mm;
                 e.v[1] = mm * x;
                 e.v[2] = mm * y;
