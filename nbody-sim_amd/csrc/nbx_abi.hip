@@ -172,7 +172,7 @@ void core_layout(const SegTable &T, Bufs &B, Take &&take)
     take(B.opar, n);
     take(B.opar_leaf, n);
     take(B.kids, 8 * (size_t)n);
-    take(B.bin_com, n);
+    take(B.brange, n);
     take(B.bin_cell, n);
     take(B.rec_a, B.n_rec);
     take(B.rec_b, B.n_rec);
@@ -232,10 +232,10 @@ void free_bufs(nbx_tree *t)
     void *ptrs[] = {B.keys[0], B.keys[1], B.vals[0], B.vals[1], B.ghist,   B.status[0],
                     B.status[1], B.tickets, B.prefix, B.scanw,
                     B.bodies,  B.left,    B.right,   B.plen,    B.oct,     B.cnt,
-                    B.first,   B.alloc,   B.bin_com, B.bin_cell, B.rec_a,  B.rec_b,
+                    B.first,   B.alloc,   B.bin_cell, B.rec_a,  B.rec_b,
                     B.rec_c,   B.partial, B.roots,   B.dv,      B.err,     B.labels,
                     B.step_ctr, B.dtab,    B.prefix2,
-                    B.bin_q,   B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
+                    B.rec_q,   B.lo,      B.diag,    B.brange,  B.rec_r,
                     B.adj_g,   B.adj_d,   B.adj_blk, B.adj_p,   B.spk,     B.spi,
                     B.bm,      B.lpt_order, B.lpt_cost, B.opar, B.opar_leaf, B.spay,
                     B.kids,    B.lpt_hist, B.s64,  B.rec64};
@@ -526,7 +526,6 @@ int nbx_tree_set_multipole(nbx_tree *t, int order)
         const int n = t->T.n_total;
         int rc;
         if ((rc = dmalloc(&B.prefix2, (size_t)n + t->T.nseg)) != NBX_OK) return rc;
-        if ((rc = dmalloc(&B.bin_q, (size_t)2 * n)) != NBX_OK) return rc;
         if ((rc = dmalloc(&B.rec_q, (size_t)2 * B.n_rec)) != NBX_OK) return rc;
         HIPCHK(hipMemset(B.rec_q, 0, sizeof(float4) * 2 * B.n_rec));
     }
@@ -563,10 +562,9 @@ int nbx_tree_set_precision(nbx_tree *t, int mode)
             return fail(NBX_E_ARG, "the FP64 step takes at most 2^24 bodies");
         int rc;
         if ((!B.s64 && (rc = dmalloc(&B.s64, (size_t)9 * n))) ||
-            (!B.rec64 && (rc = dmalloc(&B.rec64, (size_t)B.n_rec))) ||
-            (!B.brange && (rc = dmalloc(&B.brange, (size_t)n))))
+            (!B.rec64 && (rc = dmalloc(&B.rec64, (size_t)B.n_rec))))
             return rc;
-        t->stage = kNone;  // the next build records node ranges
+        t->stage = kNone;  // the FP64 moments come with the next build
         t->s64_ready = false;  // loaded from the float state by nbx_init_forces
     }
     if (t->precision != mode && t->graph) {
@@ -1266,7 +1264,7 @@ int nbx_tree_set_adjoint(nbx_tree *t, int enable)
     Bufs &B = t->B;
     const int n = t->T.n_total;
     int rc;
-    if ((!B.brange && (rc = dmalloc(&B.brange, n))) || (rc = dmalloc(&B.rec_r, B.n_rec)) ||
+    if ((rc = dmalloc(&B.rec_r, B.n_rec)) ||
         (rc = dmalloc(&B.adj_g, 3 * (size_t)B.n_rec)) ||
         (rc = dmalloc(&B.adj_d, 3 * (size_t)(n + 1))) ||
         (rc = dmalloc(&B.adj_blk, 3 * (size_t)adj_scan_blocks(n))) || (rc = dmalloc(&B.adj_p, 4)))

@@ -42,7 +42,6 @@ struct Bufs {
     int quad = 0;                // 1: quadrupole moments are built and used
     DVec<6> *prefix2 = nullptr;  // [n + 1] exclusive prefix of the second moments
     DVec<6> *pagg2 = nullptr, *pinc2 = nullptr;
-    float4 *bin_q = nullptr;     // [2 n] traceless quadrupole of binary nodes
     float4 *rec_q = nullptr;     // [2 n_rec] quadrupole of walk records (0 for leaves)
     float4 *bodies = nullptr;    // [n] sorted {x,y,z,m}
     int *left = nullptr, *right = nullptr;   // [n] binary children
@@ -52,8 +51,8 @@ struct Bufs {
     int *first = nullptr;        // [n] first child record of octree node
     int *alloc = nullptr;        // [kMaxSeg] child records per disk (statistics)
     int *maxlev = nullptr;       // [kMaxSeg] deepest octree level per disk
-    float4 *bin_com = nullptr;   // [n] {com, mass} of binary nodes
     float4 *bin_cell = nullptr;  // [n] {cell centre, half size} of binary nodes
+    int2 *brange = nullptr;      // [n] sorted-position range [lo, hi] of binary nodes
     float4 *rec_a = nullptr;     // walk records {com, mass}
     int4 *rec_b = nullptr;       // walk records {code, h^2, first, count}
     float4 *rec_c = nullptr;     // walk records {cell centre | body, h}
@@ -72,7 +71,6 @@ struct Bufs {
     int dtab_cap = 0;
     int *step_ctr = nullptr;     // [1] step index into dtab, advanced by k_kick_drag
     // reverse mode (allocated on demand)
-    int2 *brange = nullptr;      // [n] sorted-position range of binary nodes
     int2 *rec_r = nullptr;       // [n_rec] sorted-position range of walk records
     int *opar = nullptr;         // [n] octree parent (binary index) of octree nodes, -1: root
     int *opar_leaf = nullptr;    // [n] octree parent of every sorted body's leaf record

@@ -1335,11 +1335,10 @@ __host__ __device__ inline int pbase(const SegTable &T, int s) { return T.off[s]
 // the positions of the FP64 state s64 ([x | y | z] in body order, read
 // through the sorted order perm) instead of the float bodies.
 template <int N, int KIND>
-__global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__restrict__ bodies,
-                                                   DVec<N> *P, DVec<N> *pagg,
-                                                   DVec<N> *pinc, int *tickets, int slot,
-                                                   int *ghist, const double *__restrict__ s64,
-                                                   const int *__restrict__ perm)
+__device__ inline void prefix_tile(const SegTable &T, const float4 *__restrict__ bodies,
+                                   DVec<N> *P, DVec<N> *pagg, DVec<N> *pinc, int *tickets,
+                                   int slot, int *ghist, const double *__restrict__ s64,
+                                   const int *__restrict__ perm)
 {
     __shared__ int s_tile;
     __shared__ DVec<N> s_w[kBlock / kWave];
@@ -1462,10 +1461,11 @@ __global__ __launch_bounds__(kBlock) void k_prefix(SegTable T, const float4 *__r
 // prefix search, ties broken by position).  Internal node j of disk s covers
 // sorted positions [lo, hi]; children are encoded >= 0 (internal, global
 // index) or < 0 (leaf, ~global sorted position).  The same thread derives
-// the node's mass and COM from the prefix, its octree level
-// floor(prefix bits / 3), whether it starts a new octree cell (its parent's
-// prefix is max(lcp(lo-1, lo), lcp(hi, hi+1))), and the cell's centre by
-// the float descent of the keys.
+// the node's octree level floor(prefix bits / 3), whether it starts a new
+// octree cell (its parent's prefix is max(lcp(lo-1, lo), lcp(hi, hi+1))),
+// and the cell's centre by the float descent of the keys.  It needs only the
+// sorted keys, so it runs in the launch of the moment prefix (below); the
+// record pass takes each node's mass and COM from the prefix.
 // ===========================================================================
 
 __device__ inline int lcp(const uint64_t *__restrict__ k, int n, int a, int b)
@@ -1523,14 +1523,11 @@ __device__ inline int last_true(const uint64_t *__restrict__ k, int n, int i, ui
 // (prefix >= 64) form level-21 cells.  Key bit 63 is always zero.
 __device__ inline int oct_level(int plen) { return plen >= 64 ? kLevels : (plen - 1) / 3; }
 
-__global__ __launch_bounds__(kBlock, 8) void k_karras(
-    SegTable T, const uint64_t *__restrict__ keys, const DVec<4> *__restrict__ P,
-    const DVec<6> *__restrict__ P2, const float4 *__restrict__ bodies,
-    const Root *__restrict__ roots, int *left, int *right, int *plen, int *octf,
-    float4 *bin_com, float4 *bin_cell, float4 *bin_q, int2 *brange)
+__device__ inline void karras_node(const SegTable &T, const uint64_t *__restrict__ keys,
+                                   const Root *__restrict__ roots, int *left, int *right,
+                                   int *plen, int *octf, float4 *bin_cell, int2 *brange, int j)
 {
 #pragma clang fp contract(off)
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= T.ioff[T.nseg]) return;
     const int s = seg_of_internal(T, j);
     const int n = T.off[s + 1] - T.off[s];
@@ -1571,7 +1568,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_karras(
     // split: the last m in [0, l) with lcp(i, i + m d) > dnode
     const int sp = last_true(k, n, i, ki, d, 0, dnode, l);
     const int gamma = i + sp * d + min(d, 0);
-    if (brange) brange[j] = make_int2(T.off[s] + lo, T.off[s] + hi);
+    brange[j] = make_int2(T.off[s] + lo, T.off[s] + hi);
     left[j] = lo == gamma ? ~(T.off[s] + gamma) : T.ioff[s] + gamma;
     right[j] = hi == gamma + 1 ? ~(T.off[s] + gamma + 1) : T.ioff[s] + gamma + 1;
     plen[j] = dnode;
@@ -1580,36 +1577,6 @@ __global__ __launch_bounds__(kBlock, 8) void k_karras(
     const uint64_t klo = lo == i ? ki : kjj, khi = hi == i ? ki : kjj;
     const int dpar = max(lcp_kb(n, lo, klo, lo - 1, klo1), lcp_kb(n, hi, khi, hi + 1, khi1));
     octf[j] = (dpar < 0 || oct_level(dpar) != lev || lev == kLevels) ? 1 : 0;
-    // mass and COM over sorted positions [lo, hi]
-    const DVec<4> a = P[pbase(T, s) + lo], b = P[pbase(T, s) + hi + 1];
-    const double mm = b.v[0] - a.v[0];
-    double cxd = 0, cyd = 0, czd = 0;
-    float4 c;
-    if (mm > 0.0) {
-        cxd = (b.v[1] - a.v[1]) / mm;
-        cyd = (b.v[2] - a.v[2]) / mm;
-        czd = (b.v[3] - a.v[3]) / mm;
-        c = make_float4((float)cxd, (float)cyd, (float)czd, (float)mm);
-    } else {
-        const float4 p = bodies[T.off[s] + lo], q = bodies[T.off[s] + hi];
-        c = make_float4(0.5f * (p.x + q.x), 0.5f * (p.y + q.y), 0.5f * (p.z + q.z), 0.f);
-    }
-    bin_com[j] = c;
-    if (P2) {
-        // traceless quadrupole about the COM: Q = 3 C - tr(C) I with the
-        // central second moment C = S - M c c^T of the range sums S
-        const DVec<6> u = P2[pbase(T, s) + lo], o = P2[pbase(T, s) + hi + 1];
-        const double sxx = o.v[0] - u.v[0] - mm * cxd * cxd;
-        const double syy = o.v[1] - u.v[1] - mm * cyd * cyd;
-        const double szz = o.v[2] - u.v[2] - mm * czd * czd;
-        const double sxy = o.v[3] - u.v[3] - mm * cxd * cyd;
-        const double sxz = o.v[4] - u.v[4] - mm * cxd * czd;
-        const double syz = o.v[5] - u.v[5] - mm * cyd * czd;
-        const double tr = sxx + syy + szz;
-        bin_q[2 * j] = make_float4((float)(3 * sxx - tr), (float)(3 * syy - tr),
-                                   (float)(3 * szz - tr), (float)(3 * sxy));
-        bin_q[2 * j + 1] = make_float4((float)(3 * sxz), (float)(3 * syz), 0.f, 0.f);
-    }
     // cell centre: descend `lev` octant digits of this node's prefix
     const Root R = roots[s];
     float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
@@ -1622,6 +1589,74 @@ __global__ __launch_bounds__(kBlock, 8) void k_karras(
         cz = cz + ((dg & 4) ? hh : -hh);
     }
     bin_cell[j] = make_float4(cx, cy, cz, hh);
+}
+
+// One launch for two independent passes over the sorted order: the first
+// ntile blocks are the moment prefix (and the next ntile the second
+// moments, with quadrupoles), each taking its tiles by ticket in dispatch
+// order, so a tile waits only on tiles already running; the other blocks
+// build the binary radix tree, which never waits.
+template <int KIND0>
+__global__ __launch_bounds__(kBlock) void k_prefix_karras(
+    SegTable T, const float4 *__restrict__ bodies, DVec<4> *P, DVec<4> *pagg, DVec<4> *pinc,
+    DVec<6> *P2, DVec<6> *pagg2, DVec<6> *pinc2, int *tickets, int *ghist,
+    const double *__restrict__ s64, const int *__restrict__ perm, int ntile,
+    const uint64_t *__restrict__ keys, const Root *__restrict__ roots, int *left, int *right,
+    int *plen, int *octf, float4 *bin_cell, int2 *brange)
+{
+    const int b = blockIdx.x;
+    if (b < ntile) {
+        prefix_tile<4, KIND0>(T, bodies, P, pagg, pinc, tickets, kTicketScan, ghist, s64, perm);
+        return;
+    }
+    const int nscan = P2 ? 2 * ntile : ntile;
+    if (b < nscan) {
+        prefix_tile<6, 1>(T, bodies, P2, pagg2, pinc2, tickets, kTicketScan2, ghist, nullptr,
+                          nullptr);
+        return;
+    }
+    karras_node(T, keys, roots, left, right, plen, octf, bin_cell, brange,
+                (b - nscan) * kBlock + (int)threadIdx.x);
+}
+
+// Mass and COM of sorted positions [q.x, q.y] of disk s from the moment
+// prefix (divided once, rounded to float), and the traceless quadrupole
+// about the COM when P2 is given: Q = 3 C - tr(C) I with the central second
+// moment C = S - M c c^T of the range sums S.  A massless range takes the
+// midpoint of its end bodies.
+__device__ inline float4 range_moments(const SegTable &T, int s, int2 q,
+                                       const DVec<4> *__restrict__ P,
+                                       const DVec<6> *__restrict__ P2,
+                                       const float4 *__restrict__ bodies, float4 *quad)
+{
+    const int lo = q.x - T.off[s], hi = q.y - T.off[s];
+    const DVec<4> a = P[pbase(T, s) + lo], b = P[pbase(T, s) + hi + 1];
+    const double mm = b.v[0] - a.v[0];
+    double cxd = 0, cyd = 0, czd = 0;
+    float4 c;
+    if (mm > 0.0) {
+        cxd = (b.v[1] - a.v[1]) / mm;
+        cyd = (b.v[2] - a.v[2]) / mm;
+        czd = (b.v[3] - a.v[3]) / mm;
+        c = make_float4((float)cxd, (float)cyd, (float)czd, (float)mm);
+    } else {
+        const float4 p = bodies[q.x], r = bodies[q.y];
+        c = make_float4(0.5f * (p.x + r.x), 0.5f * (p.y + r.y), 0.5f * (p.z + r.z), 0.f);
+    }
+    if (P2) {
+        const DVec<6> u = P2[pbase(T, s) + lo], o = P2[pbase(T, s) + hi + 1];
+        const double sxx = o.v[0] - u.v[0] - mm * cxd * cxd;
+        const double syy = o.v[1] - u.v[1] - mm * cyd * cyd;
+        const double szz = o.v[2] - u.v[2] - mm * czd * czd;
+        const double sxy = o.v[3] - u.v[3] - mm * cxd * cyd;
+        const double sxz = o.v[4] - u.v[4] - mm * cxd * czd;
+        const double syz = o.v[5] - u.v[5] - mm * cyd * czd;
+        const double tr = sxx + syy + szz;
+        quad[0] = make_float4((float)(3 * sxx - tr), (float)(3 * syy - tr), (float)(3 * szz - tr),
+                              (float)(3 * sxy));
+        quad[1] = make_float4((float)(3 * sxz), (float)(3 * syz), 0.f, 0.f);
+    }
+    return c;
 }
 
 // Enumerate the octree children of octree node j in Morton order: the
@@ -1754,8 +1789,8 @@ __global__ __launch_bounds__(kBlock) void k_tree_stats(SegTable T, const int *__
 // data in one round trip and store its block as contiguous records.
 __global__ __launch_bounds__(kBlock) void k_oct_fill(
     SegTable T, const int *__restrict__ cnt, const int *__restrict__ first,
-    const float4 *__restrict__ bin_com, const float4 *__restrict__ bin_cell,
-    const float4 *__restrict__ bodies, const float4 *__restrict__ bin_q, float4 *rec_a,
+    const DVec<4> *__restrict__ P, const DVec<6> *__restrict__ P2,
+    const float4 *__restrict__ bin_cell, const float4 *__restrict__ bodies, float4 *rec_a,
     int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r,
     int *opar, int *opar_leaf, const int *__restrict__ kids)
 {
@@ -1765,19 +1800,12 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     auto put_r = [&](int code, int r) {
         if (rec_r) rec_r[r] = code < 0 ? make_int2(~code, ~code) : brange[code];
     };
-    auto put_q = [&](int code, int r) {
-        if (!rec_q) return;
-        if (code < 0) {
-            rec_q[2 * r] = make_float4(0.f, 0.f, 0.f, 0.f);
-            rec_q[2 * r + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        } else {
-            rec_q[2 * r] = bin_q[2 * code];
-            rec_q[2 * r + 1] = bin_q[2 * code + 1];
-        }
-    };
     // rec_b = {code, h^2, first, count}: the record's own code (binary node,
-    // or ~sorted position of a leaf) lets a search continue below any record
-    auto rec_of = [&](int code, float4 &a, int4 &b, float4 &c) {
+    // or ~sorted position of a leaf) lets a search continue below any record;
+    // a node's mass and COM (and quadrupole, into rec_q) come from the prefix
+    // over its sorted range
+    auto rec_of = [&](int s, int code, int r, float4 &a, int4 &b, float4 &c) {
+        float4 q[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
         if (code < 0) {
             const int p = ~code;
             a = bodies[p];
@@ -1785,19 +1813,22 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
             c = make_float4(a.x, a.y, a.z, 0.f);
         } else {
             c = bin_cell[code];
-            a = bin_com[code];
+            a = range_moments(T, s, brange[code], P, rec_q ? P2 : nullptr, bodies, q);
             b = make_int4(code, __float_as_int(c.w * c.w), first[code], cnt[code]);
+        }
+        if (rec_q) {
+            rec_q[2 * r] = q[0];
+            rec_q[2 * r + 1] = q[1];
         }
     };
     if (k == 0 && j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
         float4 a, c;
         int4 b;
         opar_leaf[T.off[j]] = -1;  // the body is the disk's root
-        rec_of(~T.off[j], a, b, c);
+        rec_of(j, ~T.off[j], T.noff[j], a, b, c);
         rec_a[T.noff[j]] = a;
         rec_b[T.noff[j]] = b;
         rec_c[T.noff[j]] = c;
-        put_q(~T.off[j], T.noff[j]);
         put_r(~T.off[j], T.noff[j]);
     }
     if (j >= T.ioff[T.nseg]) return;
@@ -1810,11 +1841,10 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         float4 a, c;
         int4 b;
         opar[j] = -1;
-        rec_of(j, a, b, c);
+        rec_of(s, j, T.noff[s], a, b, c);
         rec_a[T.noff[s]] = a;
         rec_b[T.noff[s]] = b;
         rec_c[T.noff[s]] = c;
-        put_q(j, T.noff[s]);
         put_r(j, T.noff[s]);
     }
     if (k >= nk) return;
@@ -1823,11 +1853,10 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     // octree parent (binary index) of every record, for bottom-up searches
     if (code < 0) opar_leaf[~code] = j;
     else opar[code] = j;
-    rec_of(code, a, b, c);
+    rec_of(s, code, f0 + k, a, b, c);
     rec_a[f0 + k] = a;
     rec_b[f0 + k] = b;
     rec_c[f0 + k] = c;
-    put_q(code, f0 + k);
     put_r(code, f0 + k);
 }
 
@@ -2037,23 +2066,15 @@ void launch_build(const SegTable &T, const float *x, const float *y, const float
     case 11: launch_sort<11>(T, x, y, z, mass, B, st); break;
     default: launch_sort<8>(T, x, y, z, mass, B, st); break;
     }
-    if (s64)
-        hipLaunchKernelGGL((k_prefix<4, 2>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
-                           B.bodies, B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist,
-                           s64, B.vals[B.cur]);
-    else
-        hipLaunchKernelGGL((k_prefix<4, 0>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
-                           B.bodies, B.prefix, B.pagg, B.pinc, B.tickets, kTicketScan, B.ghist,
-                           nullptr, nullptr);
-    if (B.quad)
-        hipLaunchKernelGGL((k_prefix<6, 1>), dim3(B.n_scan_tiles), dim3(kBlock), 0, st, T,
-                           B.bodies, B.prefix2, B.pagg2, B.pinc2, B.tickets,
-                           kTicketScan2, B.ghist, nullptr, nullptr);
+    // the moment prefix(es) and the radix tree in one launch
     const int ni = T.ioff[T.nseg];
-    if (ni > 0)
-        hipLaunchKernelGGL(k_karras, NBX_GRID(ni), T, B.keys[B.cur], B.prefix,
-                           B.quad ? B.prefix2 : nullptr, B.bodies, B.roots, B.left, B.right,
-                           B.plen, B.oct, B.bin_com, B.bin_cell, B.bin_q, B.brange);
+    const int ntile = B.n_scan_tiles;
+    const int grid = (B.quad ? 2 * ntile : ntile) + cdiv(ni, kBlock);
+    auto kern = s64 ? k_prefix_karras<2> : k_prefix_karras<0>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, T, B.bodies, B.prefix, B.pagg,
+                       B.pinc, B.quad ? B.prefix2 : nullptr, B.pagg2, B.pinc2, B.tickets, B.ghist,
+                       s64, B.vals[B.cur], ntile, B.keys[B.cur], B.roots, B.left, B.right,
+                       B.plen, B.oct, B.bin_cell, B.brange);
 }
 
 void launch_tree_stats(const SegTable &T, Bufs &B, hipStream_t st)
@@ -2071,8 +2092,9 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.cnt, B.first,
                            B.err, B.kids);
     hipLaunchKernelGGL(k_oct_fill, NBX_GRID(8 * (ni > kMaxSeg ? ni : kMaxSeg)), T, B.cnt, B.first,
-                       B.bin_com, B.bin_cell, B.bodies, B.bin_q, B.rec_a, B.rec_b, B.rec_c,
-                       B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar, B.opar_leaf, B.kids);
+                       B.prefix, B.quad ? B.prefix2 : nullptr, B.bin_cell, B.bodies, B.rec_a,
+                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar,
+                       B.opar_leaf, B.kids);
 }
 
 }  // namespace nbx
