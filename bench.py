@@ -50,15 +50,16 @@ def measured_traffic(kernel):
     return None
 
 
-BUILD_KERNELS = ["k_keys_scatter", "k_bucket_sort", "k_prefix<4,0>", "k_karras", "k_oct_count",
+BUILD_KERNELS = ["k_keys_scatter", "k_bucket_sort", "k_prefix_karras<0>", "k_oct_count",
                  "k_oct_fill"]
 
 
 def build_traffic_model(n, n_rec):
     """Compulsory HBM bytes of one tree build on the step's path (key pass
-    with bucket scatter, bucket sort, moment prefix, hierarchy, record count
-    and fill): every array read once by its consumer and written once by its
-    producer.  n_rec: walk records (n leaves plus the octree nodes)."""
+    with bucket scatter, bucket sort, moment prefix and hierarchy in one
+    launch, child count, record fill): every array read once by its consumer
+    and written once by its producer.  n_rec: walk records (n leaves plus
+    the octree nodes)."""
     n_oct = n_rec - n
     return {
         "keys: read x, y, z, m": 16 * n,
@@ -67,12 +68,12 @@ def build_traffic_model(n, n_rec):
         "bucket sort: write sorted key, index, body": 28 * n,
         "prefix: read sorted bodies": 16 * n,
         "prefix: write FP64 moments": 32 * n,
-        "hierarchy: read keys and moments": 40 * n,
-        "hierarchy: write children, prefix length, flag, COM, cell": 48 * n,
-        "count: read children, flags, prefix lengths": 16 * n,
-        "count: write counts, first child, child codes": 8 * n + 32 * n_oct,
-        "fill: read counts, first, codes, COM and cell of nodes, leaf bodies":
-            72 * n_oct + 16 * n,
+        "hierarchy: read keys": 8 * n,
+        "hierarchy: write children, prefix length, flag, cell, range": 40 * n,
+        "count: read children, flags": 12 * n,
+        "count: write counts, first child, child codes": 4 * n + 36 * n_oct,
+        "fill: read counts, first, codes, cell, range and moments of nodes, leaf bodies":
+            4 * n + 132 * n_oct + 16 * n,
         "fill: write records and octree parents": 52 * n_rec,
     }
 
