@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -70,6 +71,10 @@ struct nbx_tree {
     hipEvent_t dtab_done = nullptr;
     // captured single step: replayed once per step with the device step counter
     hipGraphExec_t graph = nullptr;
+    // float steps with the next step's first half-kick fused into the end
+    // of the step: `graph` is every step but the last, `graph_last` the last
+    hipGraphExec_t graph_last = nullptr;
+    bool fused = false;
     std::vector<unsigned char> graph_key;
     // NBX_PRECISION_*: float step, FP64-compensated KDK (B.lo holds the low
     // parts) or FP64 step (B.s64 holds the state once s64_ready)
@@ -246,7 +251,8 @@ void free_bufs(nbx_tree *t)
     if (t->h_adj) (void)hipHostFree(t->h_adj);
     if (t->dtab_done) (void)hipEventDestroy(t->dtab_done);
     if (t->graph) (void)hipGraphExecDestroy(t->graph);
-    t->graph = nullptr;
+    if (t->graph_last) (void)hipGraphExecDestroy(t->graph_last);
+    t->graph = t->graph_last = nullptr;
     B = Bufs{};
     t->h_err = nullptr;
     t->h_dtab = nullptr;
@@ -269,6 +275,21 @@ int fp64_checks(nbx_tree *t, const nbx_step_params *sp)
     if (sp->collisions) return fail(NBX_E_ARG, "the FP64 step has no collision pass");
     if (t->B.quad) return fail(NBX_E_ARG, "the FP64 step walks monopoles only");
     return NBX_OK;
+}
+
+// Both captured steps (whichever exist), for a change of what they bake in.
+hipError_t drop_graphs(nbx_tree *t)
+{
+    hipError_t e = hipSuccess;
+    for (hipGraphExec_t *g : {&t->graph, &t->graph_last}) {
+        if (*g) {
+            const hipError_t r = hipGraphExecDestroy(*g);
+            if (e == hipSuccess) e = r;
+            *g = nullptr;
+        }
+    }
+    t->graph_key.clear();
+    return e;
 }
 
 // Streams and events of a tree (whichever were created), then the tree.
@@ -384,10 +405,7 @@ int upload_steps(nbx_tree *t, const nbx_disk_params *dp, int step0, int n, float
         HIPCHK(hipMalloc((void **)&B.dtab, sizeof(DiskDev) * kMaxSeg * cap));
         HIPCHK(hipHostMalloc((void **)&t->h_dtab, sizeof(DiskDev) * kMaxSeg * cap));
         B.dtab_cap = cap;
-        if (t->graph) {  // the captured step holds the old table address
-            HIPCHK(hipGraphExecDestroy(t->graph));
-            t->graph = nullptr;
-        }
+        HIPCHK(drop_graphs(t));  // the captured step holds the old table address
     }
     if (!t->dtab_done) HIPCHK(hipEventCreateWithFlags(&t->dtab_done, hipEventDisableTiming));
     HIPCHK(hipEventSynchronize(t->dtab_done));  // the previous upload has left the staging
@@ -532,10 +550,7 @@ int nbx_tree_set_multipole(nbx_tree *t, int order)
     if (B.quad != (order == 2)) {
         B.quad = order == 2;
         t->stage = kNone;  // records must be rebuilt with (or without) moments
-        if (t->graph) {
-            HIPCHK(hipGraphExecDestroy(t->graph));
-            t->graph = nullptr;
-        }
+        HIPCHK(drop_graphs(t));
     }
     return NBX_OK;
 }
@@ -567,10 +582,7 @@ int nbx_tree_set_precision(nbx_tree *t, int mode)
         t->stage = kNone;  // the FP64 moments come with the next build
         t->s64_ready = false;  // loaded from the float state by nbx_init_forces
     }
-    if (t->precision != mode && t->graph) {
-        HIPCHK(hipGraphExecDestroy(t->graph));
-        t->graph = nullptr;
-    }
+    if (t->precision != mode) HIPCHK(drop_graphs(t));
     t->precision = mode;
     return NBX_OK;
 }
@@ -618,10 +630,7 @@ int nbx_tree_set_radix_bits(nbx_tree *t, int bits)
                 return fail(NBX_E_ARG, "splitter sort takes disks of at most %d bodies", kSplitMaxN);
     t->B.radix_bits = bits;
     t->stage = kNone;
-    if (t->graph) {
-        HIPCHK(hipGraphExecDestroy(t->graph));
-        t->graph = nullptr;
-    }
+    HIPCHK(drop_graphs(t));
     return NBX_OK;
 }
 
@@ -633,10 +642,7 @@ int nbx_tree_set_wave_order(nbx_tree *t, int order)
     t->wave_order = order;
     t->lpt = order == NBX_WAVES_LONGEST_FIRST;
     t->lpt_tg = 0;  // longest-first buffers restart from the identity when used again
-    if (t->graph) {
-        HIPCHK(hipGraphExecDestroy(t->graph));
-        t->graph = nullptr;
-    }
+    HIPCHK(drop_graphs(t));
     return NBX_OK;
 }
 
@@ -1028,32 +1034,36 @@ static void enqueue_step64(nbx_tree *t, nbx_particles *p, const nbx_step_params 
     if (e0 >= 0) ev_record(t, e0 + 5);
 }
 
-static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
-                         const DiskTable &D, int e0)
+// A float step whose end can carry the next step's first half-kick and
+// drift (k_drag_kick_drift): not with phase timing (it times the kicks
+// apart), not with the longest-first wave order (whose counting pass rides
+// on the kick-drift kernel).
+static bool fusable(const nbx_tree *t)
 {
-    if (t->precision == NBX_PRECISION_FP64) {
-        enqueue_step64(t, p, sp, e0);
-        return;
-    }
+    return t->precision == NBX_PRECISION_FLOAT && !t->lpt && !t->timing;
+}
+
+// First half-kick and drift with the boxes of the new positions (main.cu:104-107).
+static void enqueue_head(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                         const LptDev &lpt)
+{
     const float dt = sp->dt, dth = sp->dt * 0.5f;
-    const int has_drag = sp->gas_drag_rate > 0.0f;
-    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
-    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
-               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr,
-               t->lpt ? 1 : 0, t->wave_order == NBX_WAVES_XCD ? 1 : 0};
-    const bool overlap = !t->timing;
-    // the next walk's wave order, from the last walk's wave costs: counted by
-    // the kick-drift pass, placed by spare blocks of the key pass
-    const LptDev lpt = t->lpt ? lpt_dev(t->B, walk_waves(t->T, t->walk_tg), t->B.step_ctr)
-                              : LptDev{};
-    if (e0 >= 0) ev_record(t, e0 + 0);
     if (t->precision == NBX_PRECISION_COMPENSATED)
         launch_kick_drift_bbox_c(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
                                  p->az, dth, dt, t->B, t->stream, lpt);
     else
         launch_kick_drift_bbox(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay,
                                p->az, dth, dt, t->B, t->stream, lpt);
-    if (e0 >= 0) ev_record(t, e0 + 1);
+}
+
+// Rebuild, force and collision pass (main.cu:109-130).
+static void enqueue_body(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                         const DiskTable &D, int e0, const LptDev &lpt)
+{
+    WalkArgs W{sp->theta * sp->theta, sp->softening * sp->softening, sp->softening,
+               sp->star_mass, t->B.quad, t->walk_tg, p->ax, p->ay, p->az, nullptr, nullptr, nullptr,
+               t->lpt ? 1 : 0, t->wave_order == NBX_WAVES_XCD ? 1 : 0};
+    const bool overlap = !t->timing;
     launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, /*resample=*/false, lpt);
     launch_summarize(t->T, t->B, t->stream);
     if (e0 >= 0) ev_record(t, e0 + 2);
@@ -1071,17 +1081,61 @@ static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *s
         if (sp->collisions)
             launch_collide(t->T, D, p->vx, p->vy, p->vz, sp->restitution, t->B, t->stream);
     }
-    if (e0 >= 0) ev_record(t, e0 + 4);
-    if (t->precision == NBX_PRECISION_COMPENSATED)
-        launch_kick_drag_c(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
-                           sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag,
+}
+
+// Second half-kick with the collision deltas and the drag (main.cu:131-139);
+// with `next`, also the next step's first half-kick and drift (fusable trees).
+static void enqueue_tail(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp, bool next)
+{
+    const float dt = sp->dt, dth = sp->dt * 0.5f;
+    const int has_drag = sp->gas_drag_rate > 0.0f;
+    const float decay = has_drag ? expf(-sp->gas_drag_rate * sp->dt) : 1.0f;  // Integrator.cu:103
+    const float *dv = sp->collisions ? t->B.dv : nullptr;
+    if (next)
+        launch_drag_kick_drift(t->T, p->x, p->y, p->z, p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
+                               dv, dth, dt, has_drag, decay, sp->star_mass, t->B.step_ctr, t->B,
+                               t->stream);
+    else if (t->precision == NBX_PRECISION_COMPENSATED)
+        launch_kick_drag_c(p->vx, p->vy, p->vz, p->ax, p->ay, p->az, dv, p->x, p->z, dth,
+                           has_drag,
                            has_drag ? exp(-(double)sp->gas_drag_rate * (double)sp->dt) : 1.0,
                            sp->star_mass, p->n, t->B.step_ctr, t->B, t->stream);
     else
-        launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az,
-                         sp->collisions ? t->B.dv : nullptr, p->x, p->z, dth, has_drag, decay,
-                         sp->star_mass, p->n, t->B.step_ctr, t->stream);
+        launch_kick_drag(p->vx, p->vy, p->vz, p->ax, p->ay, p->az, dv, p->x, p->z, dth, has_drag,
+                         decay, sp->star_mass, p->n, t->B.step_ctr, t->stream);
+}
+
+// One whole step (timed phases when e0 >= 0).
+static void enqueue_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                         const DiskTable &D, int e0)
+{
+    if (t->precision == NBX_PRECISION_FP64) {
+        enqueue_step64(t, p, sp, e0);
+        return;
+    }
+    // the next walk's wave order, from the last walk's wave costs: counted by
+    // the kick-drift pass, placed by spare blocks of the key pass
+    const LptDev lpt = t->lpt ? lpt_dev(t->B, walk_waves(t->T, t->walk_tg), t->B.step_ctr)
+                              : LptDev{};
+    if (e0 >= 0) ev_record(t, e0 + 0);
+    enqueue_head(t, p, sp, lpt);
+    if (e0 >= 0) ev_record(t, e0 + 1);
+    enqueue_body(t, p, sp, D, e0, lpt);
+    if (e0 >= 0) ev_record(t, e0 + 4);
+    enqueue_tail(t, p, sp, false);
     if (e0 >= 0) ev_record(t, e0 + 5);
+}
+
+// n steps of a fusable tree: the head once, then every step's body and
+// tail, each tail but the last carrying the next step's head.
+static void enqueue_steps_fused(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
+                                const DiskTable &D, int n_steps)
+{
+    enqueue_head(t, p, sp, LptDev{});
+    for (int k = 0; k < n_steps; ++k) {
+        enqueue_body(t, p, sp, D, -1, LptDev{});
+        enqueue_tail(t, p, sp, k + 1 < n_steps);
+    }
 }
 
 // Everything a captured step bakes into its kernel arguments.
@@ -1125,27 +1179,66 @@ static int prepare_steps(nbx_tree *t, const nbx_step_params *sp, const nbx_disk_
     return NBX_OK;
 }
 
-// The tree's captured step (kept while nothing it bakes in changes).
+static int capture(nbx_tree *t, hipGraphExec_t *out, const std::function<void()> &enqueue)
+{
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    HIPCHK(hipStreamEndCapture(t->stream, &g));
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(NBX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return NBX_OK;
+}
+
+// The tree's captured step (kept while nothing it bakes in changes): one
+// graph of the whole step, or for a fusable tree the step whose end carries
+// the next step's head (`graph`) and the last step of a call (`graph_last`).
 static int ensure_step_graph(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
                              const DiskTable &D)
 {
+    const bool fuse = fusable(t);
     std::vector<unsigned char> key = step_key(p, sp, D, t->B, t->walk_tg);
+    key.push_back(fuse ? 1 : 0);
     if (t->graph && key == t->graph_key) return NBX_OK;
-    if (t->graph) {
-        HIPCHK(hipGraphExecDestroy(t->graph));
-        t->graph = nullptr;
+    HIPCHK(drop_graphs(t));
+    int rc;
+    if (fuse) {
+        rc = capture(t, &t->graph, [&] {
+            enqueue_body(t, p, sp, D, -1, LptDev{});
+            enqueue_tail(t, p, sp, true);
+        });
+        if (!rc)
+            rc = capture(t, &t->graph_last, [&] {
+                enqueue_body(t, p, sp, D, -1, LptDev{});
+                enqueue_tail(t, p, sp, false);
+            });
+    } else {
+        rc = capture(t, &t->graph, [&] { enqueue_step(t, p, sp, D, -1); });
     }
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(t, p, sp, D, -1);
-    HIPCHK(hipStreamEndCapture(t->stream, &g));
-    hipError_t e = hipGraphInstantiate(&t->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        t->graph = nullptr;
-        return fail(NBX_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    if (rc) {
+        (void)drop_graphs(t);
+        return rc;
     }
+    t->fused = fuse;
     t->graph_key = key;
+    return NBX_OK;
+}
+
+// Step k of n_steps of a tree whose graphs are ready (the head before step
+// 0 when the steps are fused).
+static int launch_graph_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp, int k,
+                             int n_steps)
+{
+    if (!t->fused) {
+        HIPCHK(hipGraphLaunch(t->graph, t->stream));
+        return NBX_OK;
+    }
+    if (k == 0) enqueue_head(t, p, sp, LptDev{});
+    HIPCHK(hipGraphLaunch(k + 1 < n_steps ? t->graph : t->graph_last, t->stream));
     return NBX_OK;
 }
 
@@ -1164,7 +1257,10 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
     if (graph) {
         rc = ensure_step_graph(t, p, sp, D);
         if (rc) return rc;
-        for (int k = 0; k < n_steps; ++k) HIPCHK(hipGraphLaunch(t->graph, t->stream));
+        for (int k = 0; k < n_steps; ++k)
+            if ((rc = launch_graph_step(t, p, sp, k, n_steps))) return rc;
+    } else if (fusable(t)) {
+        enqueue_steps_fused(t, p, sp, D, n_steps);
     } else {
         const int nev = 6 * n_steps;
         if (t->timing && (int)t->ev.size() < nev) {
@@ -1236,8 +1332,10 @@ int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_
         if (rc) return rc;
     }
     for (int k = 0; k < n_steps; ++k)
-        for (int i = 0; i < n_trees; ++i)
-            HIPCHK(hipGraphLaunch(trees[i]->graph, trees[i]->stream));
+        for (int i = 0; i < n_trees; ++i) {
+            const int rc = launch_graph_step(trees[i], ps[i], sp, k, n_steps);
+            if (rc) return rc;
+        }
     for (int i = 1; i < n_trees; ++i) {
         HIPCHK(hipEventRecord(trees[i]->gord, trees[i]->stream));
         HIPCHK(hipStreamWaitEvent(lead->stream, trees[i]->gord, 0));
@@ -1271,10 +1369,7 @@ int nbx_tree_set_adjoint(nbx_tree *t, int enable)
         return rc;
     HIPCHK(hipMemset(B.rec_r, 0, sizeof(int2) * B.n_rec));
     HIPCHK(hipHostMalloc((void **)&t->h_adj, 4 * sizeof(double)));
-    if (t->graph) {  // the captured step holds the build without range outputs
-        HIPCHK(hipGraphExecDestroy(t->graph));
-        t->graph = nullptr;
-    }
+    HIPCHK(drop_graphs(t));  // the captured step holds the build without range outputs
     t->adjoint = true;
     return NBX_OK;
 }

@@ -120,6 +120,15 @@ void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const fl
                       float dt_half, int has_drag, float decay, float star_mass, int n,
                       int *step_ctr, hipStream_t st);
 
+// The second half-kick with the collision deltas and the drag of one step
+// followed by the first half-kick, drift and boxes of the next (the same
+// acceleration serves both kicks); step_ctr (optional) advances by one.
+void launch_drag_kick_drift(const SegTable &T, float *x, float *y, float *z, float *vx,
+                            float *vy, float *vz, const float *ax, const float *ay,
+                            const float *az, const float *dv, float dt_half, float dt,
+                            int has_drag, float decay, float star_mass, int *step_ctr, Bufs &B,
+                            hipStream_t st);
+
 // Compensated variants: the float state holds round(X), lo[] holds
 // X - round(X); updates are evaluated in FP64 on X = hi + lo.
 void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,

@@ -149,6 +149,55 @@ __global__ void k_kick_drag(float *vx, float *vy, float *vz, const float *__rest
     vz[i] = c;
 }
 
+// The end of one step and the start of the next in one pass: v += dv;
+// v += a dt_half; drag (main.cu:131-139), then v += a dt_half; x += v dt
+// (main.cu:104-107, the same acceleration: no force pass lies between) and
+// the per-block boxes of the new positions.  Every value rounds as in the
+// two kernels it replaces.  step_ctr (optional) advances by one.
+__global__ __launch_bounds__(kBlock) void k_drag_kick_drift(
+    SegTable T, float *x, float *y, float *z, float *vx, float *vy, float *vz,
+    const float *__restrict__ ax, const float *__restrict__ ay, const float *__restrict__ az,
+    const float *__restrict__ dv, float dt_half, float dt, int has_drag, float decay,
+    float star_mass, int *step_ctr, float *partial)
+{
+#pragma clang fp contract(off)
+    if (step_ctr && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(step_ctr, 1);
+    const int b = blockIdx.x;
+    const int s = seg_of_block(T, b);
+    const int i = T.off[s] + (b - T.blk[s]) * kBlock + threadIdx.x;
+    const int n = T.n_total;
+    float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
+    float hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+    if (i < T.off[s + 1]) {
+        const float aX = ax[i], aY = ay[i], aZ = az[i];
+        float u = vx[i], v = vy[i], w = vz[i];
+        float X = x[i], Y = y[i], Z = z[i];
+        if (dv) {
+            u += dv[i];
+            v += dv[n + i];
+            w += dv[2 * n + i];
+        }
+        if (dt_half != 0.f) {  // as k_kick_drag
+            u += aX * dt_half;
+            v += aY * dt_half;
+            w += aZ * dt_half;
+        }
+        if (has_drag) gas_drag_one(X, Z, u, v, w, decay, star_mass);
+        u = u + aX * dt_half;
+        v = v + aY * dt_half;
+        w = w + aZ * dt_half;
+        X = X + u * dt;
+        Y = Y + v * dt;
+        Z = Z + w * dt;
+        vx[i] = u; vy[i] = v; vz[i] = w;
+        x[i] = X; y[i] = Y; z[i] = Z;
+        lx = hx = X;
+        ly = hy = Y;
+        lz = hz = Z;
+    }
+    bbox_post(lx, ly, lz, hx, hy, hz, partial);
+}
+
 // ---------------------------------------------------------------------------
 // Compensated mode (no reference counterpart): the same KDK / drag sequence
 // evaluated in FP64 on X = hi + lo, storing hi = round-to-float(X) in the
@@ -1917,6 +1966,17 @@ void launch_kick_drag(float *vx, float *vy, float *vz, const float *ax, const fl
 {
     hipLaunchKernelGGL(k_kick_drag, NBX_GRID(n), vx, vy, vz, ax, ay, az, dv, x, z, dt_half,
                        has_drag, decay, star_mass, n, step_ctr);
+}
+
+void launch_drag_kick_drift(const SegTable &T, float *x, float *y, float *z, float *vx,
+                            float *vy, float *vz, const float *ax, const float *ay,
+                            const float *az, const float *dv, float dt_half, float dt,
+                            int has_drag, float decay, float star_mass, int *step_ctr, Bufs &B,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_drag_kick_drift, dim3(T.blk[T.nseg]), dim3(kBlock), 0, st, T, x, y, z,
+                       vx, vy, vz, ax, ay, az, dv, dt_half, dt, has_drag, decay, star_mass,
+                       step_ctr, B.partial);
 }
 
 void launch_kick_drift_bbox_c(const SegTable &T, float *x, float *y, float *z, float *vx,
