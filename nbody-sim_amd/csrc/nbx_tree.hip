@@ -1669,17 +1669,16 @@ __global__ __launch_bounds__(kBlock) void k_prefix_karras(
 }
 
 // Mass and COM of sorted positions [q.x, q.y] of disk s from the moment
-// prefix (divided once, rounded to float), and the traceless quadrupole
-// about the COM when P2 is given: Q = 3 C - tr(C) I with the central second
-// moment C = S - M c c^T of the range sums S.  A massless range takes the
-// midpoint of its end bodies.
-__device__ inline float4 range_moments(const SegTable &T, int s, int2 q,
-                                       const DVec<4> *__restrict__ P,
-                                       const DVec<6> *__restrict__ P2,
-                                       const float4 *__restrict__ bodies, float4 *quad)
+// prefix entries at the range's start (a) and past its end (b), divided once
+// and rounded to float, and the traceless quadrupole about the COM when P2
+// is given: Q = 3 C - tr(C) I with the central second moment C = S - M c c^T
+// of the range sums S.  A massless range takes the midpoint of its end
+// bodies.
+__device__ inline float4 range_moments_of(const SegTable &T, int s, int2 q, const DVec<4> &a,
+                                          const DVec<4> &b, const DVec<6> *__restrict__ P2,
+                                          const float4 *__restrict__ bodies, float4 *quad)
 {
     const int lo = q.x - T.off[s], hi = q.y - T.off[s];
-    const DVec<4> a = P[pbase(T, s) + lo], b = P[pbase(T, s) + hi + 1];
     const double mm = b.v[0] - a.v[0];
     double cxd = 0, cyd = 0, czd = 0;
     float4 c;
@@ -1834,79 +1833,106 @@ __global__ __launch_bounds__(kBlock) void k_tree_stats(SegTable T, const int *__
 
 // Walk records: the children of octree node j into its block, the root
 // record of each disk at noff[s], and one-body disks (root = the body).
-// One thread per (node, child slot): the lanes of a node load its children's
-// data in one round trip and store its block as contiguous records.
+// A thread takes kFillSlots child slots of one node and reads in three
+// rounds for all of them at once: the node's count and child codes; each
+// child's cell, range and count (a leaf: its body); the prefix entries at
+// the ends of each child's range.  Its records are then stored together.
+constexpr int kFillSlots = 2;
+
 __global__ __launch_bounds__(kBlock) void k_oct_fill(
-    SegTable T, const int *__restrict__ cnt, const int *__restrict__ first,
-    const DVec<4> *__restrict__ P, const DVec<6> *__restrict__ P2,
-    const float4 *__restrict__ bin_cell, const float4 *__restrict__ bodies, float4 *rec_a,
-    int4 *rec_b, float4 *rec_c, float4 *rec_q, const int2 *__restrict__ brange, int2 *rec_r,
-    int *opar, int *opar_leaf, const int *__restrict__ kids)
+    SegTable T, const int *__restrict__ cnt, const DVec<4> *__restrict__ P,
+    const DVec<6> *__restrict__ P2, const float4 *__restrict__ bin_cell,
+    const float4 *__restrict__ bodies, float4 *__restrict__ rec_a, int4 *__restrict__ rec_b,
+    float4 *__restrict__ rec_c, float4 *__restrict__ rec_q, const int2 *__restrict__ brange,
+    int2 *__restrict__ rec_r, int *__restrict__ opar, int *__restrict__ opar_leaf,
+    const int *__restrict__ kids)
 {
+    constexpr int SL = kFillSlots, PER = 8 / SL;
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = tid >> 3, k = tid & 7;
-    // sorted-position range [lo, hi] covered by a record (adjoint only)
-    auto put_r = [&](int code, int r) {
-        if (rec_r) rec_r[r] = code < 0 ? make_int2(~code, ~code) : brange[code];
-    };
-    // rec_b = {code, h^2, first, count}: the record's own code (binary node,
-    // or ~sorted position of a leaf) lets a search continue below any record;
-    // a node's mass and COM (and quadrupole, into rec_q) come from the prefix
-    // over its sorted range
-    auto rec_of = [&](int s, int code, int r, float4 &a, int4 &b, float4 &c) {
+    const int j = tid / PER, k0 = (tid % PER) * SL;
+    const int ni = T.ioff[T.nseg];
+    // the record of binary node (or ~sorted position) `code` of disk s at r
+    auto put = [&](int s, int code, int r, const float4 &cell, int2 rg, int nc, const DVec<4> &pa,
+                   const DVec<4> &pb, const float4 &body) {
+        float4 a, c;
+        int4 b;
         float4 q[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
         if (code < 0) {
-            const int p = ~code;
-            a = bodies[p];
-            b = make_int4(code, __float_as_int(-1.f), p, 0);
+            a = body;
+            b = make_int4(code, __float_as_int(-1.f), ~code, 0);
             c = make_float4(a.x, a.y, a.z, 0.f);
         } else {
-            c = bin_cell[code];
-            a = range_moments(T, s, brange[code], P, rec_q ? P2 : nullptr, bodies, q);
-            b = make_int4(code, __float_as_int(c.w * c.w), first[code], cnt[code]);
+            c = cell;
+            a = range_moments_of(T, s, rg, pa, pb, rec_q ? P2 : nullptr, bodies, q);
+            const int fc = T.noff[s] + 8 + 8 * (code - T.ioff[s]);
+            b = make_int4(code, __float_as_int(c.w * c.w), fc, nc);
         }
+        rec_a[r] = a;
+        rec_b[r] = b;
+        rec_c[r] = c;
         if (rec_q) {
             rec_q[2 * r] = q[0];
             rec_q[2 * r + 1] = q[1];
         }
+        if (rec_r) rec_r[r] = code < 0 ? make_int2(~code, ~code) : rg;
     };
-    if (k == 0 && j < T.nseg && T.off[j + 1] - T.off[j] == 1) {
-        float4 a, c;
-        int4 b;
-        opar_leaf[T.off[j]] = -1;  // the body is the disk's root
-        rec_of(j, ~T.off[j], T.noff[j], a, b, c);
-        rec_a[T.noff[j]] = a;
-        rec_b[T.noff[j]] = b;
-        rec_c[T.noff[j]] = c;
-        put_r(~T.off[j], T.noff[j]);
+    // a disk's root record: its node 0, or its only body
+    if (k0 == 0 && j < T.nseg) {
+        const bool one = T.off[j + 1] - T.off[j] == 1;
+        const int code = one ? ~T.off[j] : T.ioff[j];
+        const int ic = one ? 0 : code;
+        const int2 rg = one ? make_int2(T.off[j], T.off[j]) : brange[ic];
+        const int nc = one ? 0 : cnt[ic];
+        const float4 cell = bin_cell[ic], body = bodies[T.off[j]];
+        const int base = T.off[j] + j;  // the disk's first prefix entry
+        const DVec<4> pa = P[base + rg.x - T.off[j]], pb = P[base + rg.y - T.off[j] + 1];
+        if (one) opar_leaf[T.off[j]] = -1;
+        else opar[code] = -1;
+        put(j, code, T.noff[j], cell, rg, nc, pa, pb, body);
     }
-    if (j >= T.ioff[T.nseg]) return;
+    if (j >= ni) return;
+    // round 1: the node's count and its child codes
     const int nk = cnt[j];
-    const int code = kids[8 * j + k];  // read beside the count (no dependence)
-    const int f0 = first[j];
-    if (nk == 0) return;
+    int code[SL];
+#pragma unroll
+    for (int u = 0; u < SL; ++u) code[u] = kids[8 * j + k0 + u];
+    if (nk <= k0) return;
     const int s = seg_of_internal(T, j);
-    if (k == 0 && j == T.ioff[s]) {  // local node 0 is the disk's root
-        float4 a, c;
-        int4 b;
-        opar[j] = -1;
-        rec_of(s, j, T.noff[s], a, b, c);
-        rec_a[T.noff[s]] = a;
-        rec_b[T.noff[s]] = b;
-        rec_c[T.noff[s]] = c;
-        put_r(j, T.noff[s]);
+    const int f0 = T.noff[s] + 8 + 8 * (j - T.ioff[s]);
+    const int n_s = T.off[s + 1] - T.off[s];
+    // round 2: every child's cell, range and count, or its body (slots past
+    // the count read safe places and store nothing)
+    float4 cell[SL], body[SL];
+    int2 rg[SL];
+    int nc[SL];
+#pragma unroll
+    for (int u = 0; u < SL; ++u) {
+        const bool ok = k0 + u < nk;
+        const int ic = ok && code[u] >= 0 ? code[u] : T.ioff[s];
+        const int lp = ok && code[u] < 0 ? ~code[u] : T.off[s];
+        cell[u] = bin_cell[ic];
+        rg[u] = brange[ic];
+        nc[u] = cnt[ic];
+        body[u] = bodies[lp];
     }
-    if (k >= nk) return;
-    float4 a, c;
-    int4 b;
-    // octree parent (binary index) of every record, for bottom-up searches
-    if (code < 0) opar_leaf[~code] = j;
-    else opar[code] = j;
-    rec_of(s, code, f0 + k, a, b, c);
-    rec_a[f0 + k] = a;
-    rec_b[f0 + k] = b;
-    rec_c[f0 + k] = c;
-    put_r(code, f0 + k);
+    // round 3: the prefix at both ends of every child's range
+    DVec<4> pa[SL], pb[SL];
+    const int base = T.off[s] + s;
+#pragma unroll
+    for (int u = 0; u < SL; ++u) {
+        const int lo = min(max(rg[u].x - T.off[s], 0), n_s - 1);
+        const int hi = min(max(rg[u].y - T.off[s], 0), n_s - 1);
+        pa[u] = P[base + lo];
+        pb[u] = P[base + hi + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < SL; ++u) {
+        if (k0 + u >= nk) break;
+        // octree parent (binary index) of every record, for bottom-up searches
+        if (code[u] < 0) opar_leaf[~code[u]] = j;
+        else opar[code[u]] = j;
+        put(s, code[u], f0 + k0 + u, cell[u], rg[u], nc[u], pa[u], pb[u], body[u]);
+    }
 }
 
 // ===========================================================================
@@ -2151,10 +2177,10 @@ void launch_summarize(const SegTable &T, Bufs &B, hipStream_t st)
     if (ni > 0)
         hipLaunchKernelGGL(k_oct_count, NBX_GRID(ni), T, B.left, B.right, B.oct, B.cnt, B.first,
                            B.err, B.kids);
-    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(8 * (ni > kMaxSeg ? ni : kMaxSeg)), T, B.cnt, B.first,
-                       B.prefix, B.quad ? B.prefix2 : nullptr, B.bin_cell, B.bodies, B.rec_a,
-                       B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r, B.opar,
-                       B.opar_leaf, B.kids);
+    hipLaunchKernelGGL(k_oct_fill, NBX_GRID(8 / kFillSlots * (ni > kMaxSeg ? ni : kMaxSeg)), T,
+                       B.cnt, B.prefix, B.quad ? B.prefix2 : nullptr, B.bin_cell, B.bodies,
+                       B.rec_a, B.rec_b, B.rec_c, B.quad ? B.rec_q : nullptr, B.brange, B.rec_r,
+                       B.opar, B.opar_leaf, B.kids);
 }
 
 }  // namespace nbx
