@@ -943,11 +943,38 @@ __device__ inline int block_excl_scan(int v, int *total)
 // and the lower lane of every pair keeps the smaller element.  Elements are
 // distinct except padding, and equal elements may be exchanged freely, so
 // the upper lane takes the partner exactly when the lower one does not.
+// The value of lane (lane ^ mask).  Partners inside a row of 16 lanes come
+// over DPP at ALU speed (quad permutes, the row and half-row mirrors, row
+// shifts by 4 and 8 selected by the lane's bit); the others over the lane
+// shuffle, which goes through the LDS crossbar.  After unrolling, mask is a
+// constant and the switch folds away.
+__device__ __forceinline__ int xor_lane(int v, int mask, int lane)
+{
+    switch (mask) {
+    case 1: return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    case 3: return __builtin_amdgcn_update_dpp(v, v, 0x1B, 0xf, 0xf, false);   // quad_perm [3,2,1,0]
+    case 7: return __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false);  // row_half_mirror
+    case 15: return __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false); // row_mirror
+    case 4: {
+        const int up = __builtin_amdgcn_update_dpp(v, v, 0x104, 0xf, 0xf, false);  // row_shl:4
+        const int dn = __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+        return (lane & 4) ? dn : up;
+    }
+    case 8: {
+        const int up = __builtin_amdgcn_update_dpp(v, v, 0x108, 0xf, 0xf, false);  // row_shl:8
+        const int dn = __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+        return (lane & 8) ? dn : up;
+    }
+    default: return __shfl_xor(v, mask, kWave);
+    }
+}
+
 __device__ __forceinline__ void wave_cmpx(uint64_t &k, int &id, int lane, int mask, int hib)
 {
-    const unsigned plo = (unsigned)__shfl_xor((int)(unsigned)k, mask, kWave);
-    const unsigned phi = (unsigned)__shfl_xor((int)(unsigned)(k >> 32), mask, kWave);
-    const int pid = __shfl_xor(id, mask, kWave);
+    const unsigned plo = (unsigned)xor_lane((int)(unsigned)k, mask, lane);
+    const unsigned phi = (unsigned)xor_lane((int)(unsigned)(k >> 32), mask, lane);
+    const int pid = xor_lane(id, mask, lane);
     const uint64_t pk = ((uint64_t)phi << 32) | plo;
     if (kv_less(pk, pid, k, id) == ((lane & hib) == 0)) {
         k = pk;
