@@ -1656,7 +1656,7 @@ __device__ inline void karras_node(const SegTable &T, const uint64_t *__restrict
     // cell centre: descend `lev` octant digits of this node's prefix
     const Root R = roots[s];
     float cx = R.c[0], cy = R.c[1], cz = R.c[2], hh = R.c[3];
-    const uint64_t key = k[lo];
+    const uint64_t key = lo == i ? ki : kjj;  // k[lo], already read
     for (int q = 0; q < lev; ++q) {
         const int dg = (int)((key >> (3 * (kLevels - 1 - q))) & 7u);
         hh = hh * 0.5f;
