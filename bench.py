@@ -632,9 +632,11 @@ def diff_window(nbx, np, torch, make_disk, args):
     collisions, gravity + drag 1e-3 + 10 M_Jup perturber at 5.2 AU, 200
     steps with a checkpoint every 20; loss = mean e^2 + i^2 of the bodies in
     1-2 AU at the end.  Times the forward window and forward + reverse, and
-    checks d loss / d(perturber mass, radius) against central differences
-    (relative step 1e-2) of the forward window."""
-    from nbx.window import WindowConfig, annulus_mask, run_window, window, window_loss
+    checks d loss / d(perturber mass, radius, v0 along a random direction)
+    against central differences (relative step 1e-4) of the same window run
+    through the FP64 step."""
+    from nbx.window import (WindowConfig, annulus_mask, run_window, run_window64, window,
+                            window_loss)
     n = 32768
     d = make_disk(n, DISK_MASS, 0.5, 4.0, seed=77)
     x0 = torch.tensor(np.stack([d["x"], d["y"], d["z"]]), device="cuda").contiguous()
@@ -666,18 +668,26 @@ def diff_window(nbx, np, torch, make_disk, args):
     torch.cuda.synchronize()
     fb_ms = (time.perf_counter() - t0) * 1e3
 
-    def loss_at(pmv, rpv):
-        x, v, _ = run_window(x0, v0, pmv, rpv, m, t, cfg)
+    def loss_at(pmv, rpv, vv=v0):
+        # the FP64 step's rounding (~1e-16) leaves a relative step of 1e-4
+        # clean; the float window's (~1e-7 per step) would not
+        x, v = run_window64(x0, vv, pmv, rpv, m, t, cfg)
         return window_loss(x, v, mask).item()
 
-    h = 1e-2
+    h = 1e-4
     fd_m = (loss_at(10.0 * (1 + h), 5.2) - loss_at(10.0 * (1 - h), 5.2)) / (2 * h * 10.0)
     fd_r = (loss_at(10.0, 5.2 * (1 + h)) - loss_at(10.0, 5.2 * (1 - h))) / (2 * h * 5.2)
+    dv = torch.randn(3, n, generator=torch.Generator(device="cuda").manual_seed(3),
+                     device="cuda")
+    fd_v = (loss_at(10.0, 5.2, v0 + h * dv) - loss_at(10.0, 5.2, v0 - h * dv)) / (2 * h)
     return {"n": n, "steps": cfg.n_steps, "checkpoint_every": cfg.checkpoint_every,
             "bodies_in_loss": int(mask.sum().item()), "loss": L.item(),
             "forward_ms": fwd_ms, "forward_reverse_ms": fb_ms,
+            "fd": "central, relative step 1e-4, forward through the FP64 step",
             "dL_dmass_mjup": pm.grad.item(), "fd_dmass_mjup": fd_m,
             "dL_dradius_au": rp.grad.item(), "fd_dradius_au": fd_r,
+            "dL_dv0_along_random_dir": (v0g.grad.double() * dv.double()).sum().item(),
+            "fd_dv0_along_random_dir": fd_v,
             "dL_dv0_norm": torch.linalg.norm(v0g.grad.double()).item()}
 
 

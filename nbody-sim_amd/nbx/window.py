@@ -21,7 +21,7 @@ from dataclasses import dataclass
 import torch
 
 from . import (MJUP_TO_MSUN, ParticleData, _check, _disk_array, disk_params, init_forces, lib,
-               makePerturber, step, step_params)
+               makePerturber, set_precision, state64, step, step_params)
 
 
 @dataclass
@@ -84,6 +84,31 @@ def run_window(x0, v0, pert_mass_mjup, pert_radius_au, mass, tree, cfg, keep=Non
             keep.append((x.clone(), v.clone(), a.clone()))
         step(tree, S, sp, dp, s0, min(k, cfg.n_steps - s0))
     return x, v, a
+
+
+def run_window64(x0, v0, pert_mass_mjup, pert_radius_au, mass, tree, cfg):
+    """The same forward window through the all-FP64 step (FP64 positions,
+    velocities, moments and forces; the octree from the rounded positions):
+    returns the final FP64 state (x, v), each (3, N) float64.  Its rounding
+    (~1e-16 relative) leaves central differences with a relative step of 1e-4
+    clean, where the float window's rounding (~1e-7 per step) does not.  The
+    tree is switched to the FP64 step for the run and back to the float step
+    afterwards."""
+    _check_inputs(x0, v0, mass, tree)
+    sp, dp = _params(cfg, pert_mass_mjup, pert_radius_au)
+    n = x0.shape[1]
+    ids = torch.arange(n, dtype=torch.int32, device=x0.device)
+    x = x0.detach().clone().contiguous()
+    v = v0.detach().clone().contiguous()
+    S = _state(x, v, torch.zeros_like(x), mass, ids)
+    set_precision(tree, "fp64")
+    try:
+        init_forces(tree, S, sp, dp)  # loads the FP64 state from x0, v0
+        step(tree, S, sp, dp, 0, cfg.n_steps)
+        s = state64(tree, S)
+    finally:
+        set_precision(tree, "float")
+    return s[0:3].clone(), s[3:6].clone()
 
 
 class DiskWindow(torch.autograd.Function):
