@@ -78,6 +78,12 @@ def build_traffic_model(n, n_rec):
     }
 
 
+def progress(msg):
+    """One progress line on stderr per finished leg (stdout holds only the
+    JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,6 +315,7 @@ def main():
     nbx.sync(t)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    progress(f"timed loop: {args.steps} steps in {elapsed:.3f} s")
     # FoF at every 100th step (main.cu:143-148 with output every 100 steps):
     # timed on its own and amortised into every step, so a window of any
     # length carries one FoF per 100 steps
@@ -400,17 +407,24 @@ def main():
                        "tuned_ms": {str(k): v for k, v in tune_ms.items()}},
         "force_err": acc_stats,
     }
+    progress(f"headline done: {value / 1e6:.1f} M particle-steps/s")
     if not args.no_accurate:
         out["accurate_step"] = accurate_step(nbx, np, torch, make_disk, args, fof_ms)
+        progress("accurate step done")
     if not args.no_batch:
         out["batched_sweep"] = batched_sweep(nbx, np, torch, make_disk, args)
+        progress("batched sweep done")
     if not args.no_window:
         out["diff_window"] = diff_window(nbx, np, torch, make_disk, args)
+        progress("differentiable window done")
     if not args.no_precision:
         out["precision_study"] = precision_study(nbx, np, torch, make_disk, args)
+        progress("precision study done")
     if not args.no_cpu:
         out["cpu_fof"] = fof_timing(nbx, np, torch, make_disk)
+        progress("CPU FoF done")
         out["cpu_baseline"], out["cpu_legs"] = cpu_baseline(n, r_coll, 10.0, make_disk, np)
+        progress("CPU baseline done")
     print(json.dumps(out))
 
 
