@@ -375,20 +375,27 @@ struct GravityVisitor {
         // the pull is selected, not the mass: without softening the own
         // leaf has rinv = inf and m rinv^3 would be 0 inf = NaN
         const float f = take ? a.w * rinv3 : 0.f;
-        fx += f * dx;
-        fy += f * dy;
-        fz += f * dz;
-        if constexpr (QUAD) {
+        if constexpr (!QUAD) {
+            fx += f * dx;
+            fy += f * dy;
+            fz += f * dz;
+        } else {
             // a += -Q R / r^5 + (5/2) (R.Q R) R / r^7 for R = COM - body
-            // (Q traceless; zero for leaves)
+            // (Q traceless; zero for leaves); the radial parts of the
+            // monopole and the quadrupole share one multiply-add per axis
             const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
             const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
             const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
-            const float rinv5 = take ? rinv3 * rinv2 : 0.f;
-            const float g = take ? 2.5f * (dx * qx + dy * qy + dz * qz) * rinv5 * rinv2 : 0.f;
-            fx += g * dx - rinv5 * qx;
-            fy += g * dy - rinv5 * qy;
-            fz += g * dz - rinv5 * qz;
+            const float r5 = take ? rinv3 * rinv2 : 0.f;
+            // (without softening the own leaf has rinv2 = inf: select, 0 inf = NaN)
+            const float r7 = SELF ? (take ? r5 * rinv2 : 0.f) : r5 * rinv2;
+            const float h = __builtin_fmaf(2.5f * r7, dx * qx + dy * qy + dz * qz, f);
+            fx += h * dx;
+            fy += h * dy;
+            fz += h * dz;
+            fx -= r5 * qx;
+            fy -= r5 * qy;
+            fz -= r5 * qz;
         }
         // the walk takes the open mask as ballot(mine) & ~ballot(acc)
         return acc;
