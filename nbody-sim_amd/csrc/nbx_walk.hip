@@ -660,6 +660,7 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     const int waves = M.wave[T.nseg];
     // with one target per lane a lane evaluates kUnroll64 records per step
     constexpr int U = TG >= 16 ? kUnroll64 : 1;
+    constexpr int UQ = 1;  // quadrupole records: one per lane and step
     const bool lpt = W.lpt && step_ctr && !W.xcd;
     const int chunk = W.xcd ? cdiv(waves, kXcds) : 0;
     const int blocks = W.xcd ? kXcds * cdiv(chunk, kGravWaves) : cdiv(waves, kGravWaves);
@@ -670,17 +671,17 @@ static void launch_walk_tg(const SegTable &T, const DiskDev *dtab, const int *st
     // the own-leaf test is needed only without softening
     const bool self = W.eps2 < kNoSelfTestEps2;
     if (!C) {
-        auto kern = !self ? (W.quad ? k_walk<TG, true, U, false> : k_walk<TG, false, U, false>)
-                          : (W.quad ? k_walk<TG, true, U, true> : k_walk<TG, false, U, true>);
+        auto kern = !self ? (W.quad ? k_walk<TG, true, UQ, false> : k_walk<TG, false, U, false>)
+                          : (W.quad ? k_walk<TG, true, UQ, true> : k_walk<TG, false, U, true>);
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kGravBlock), 0, st, T, M, A);
         return;
     }
     // collision blocks: a multiple of the XCD count, so the walk blocks keep
     // their die assignment
     const int ncb = kXcds * cdiv(cdiv(T.wave[T.nseg], kGravWaves * kCollideRounds), kXcds);
-    auto kern = !self ? (W.quad ? k_walk_collide<TG, true, U, false>
+    auto kern = !self ? (W.quad ? k_walk_collide<TG, true, UQ, false>
                                 : k_walk_collide<TG, false, U, false>)
-                      : (W.quad ? k_walk_collide<TG, true, U, true>
+                      : (W.quad ? k_walk_collide<TG, true, UQ, true>
                                 : k_walk_collide<TG, false, U, true>);
     hipLaunchKernelGGL(kern, dim3(ncb + blocks), dim3(kGravBlock), 0, st, T, M, A, ncb, *C,
                        search_bufs(B));
