@@ -207,7 +207,8 @@ int nbx_step(nbx_tree *t, nbx_particles *p, const nbx_step_params *sp,
  * nbx_step gives it alone.  The group starts after the work already on
  * trees[0]'s stream, and that stream waits for the whole group; without
  * use_graph, or with phase timing on any tree, the trees are stepped one
- * after the other.  A tree may appear once. */
+ * after the other.  A tree may appear once, and no two trees may step the
+ * same particle arrays (NBX_E_ARG: they would race). */
 #define NBX_MAX_GROUP 8
 int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_t n_trees,
                         const nbx_step_params *sp, const nbx_disk_params *const *dps,

@@ -1304,8 +1304,12 @@ int nbx_step_concurrent(nbx_tree *const *trees, nbx_particles *const *ps, int32_
         int rc = check_tree_particles(trees[i], ps[i]);
         if (rc) return rc;
         if (!dps[i]) return fail(NBX_E_ARG, "null disk parameters of tree %d", i);
-        for (int j = 0; j < i; ++j)
+        for (int j = 0; j < i; ++j) {
             if (trees[j] == trees[i]) return fail(NBX_E_ARG, "tree %d appears twice", i);
+            // the trees run side by side: two of them on one state would race
+            if (ps[j]->x == ps[i]->x || ps[j]->vx == ps[i]->vx || ps[j]->ax == ps[i]->ax)
+                return fail(NBX_E_ARG, "trees %d and %d step the same particle arrays", j, i);
+        }
     }
     nbx_tree *lead = trees[0];
     HIPCHK(hipEventRecord(lead->gord, lead->stream));

@@ -539,6 +539,13 @@ def test_argument_errors_raise(gpu):
     t2 = nbx.allocateOctree(100)
     with pytest.raises(RuntimeError, match="before"):
         nbx.launchForceKernel(t2, p, 0.5, 1e-3, 1.0)
+    # two trees stepped side by side on one state would race
+    t3 = nbx.allocateOctree(100)
+    sp = nbx.step_params(dt=0.01, theta=0.5, softening=1e-3, star_mass=1.0, restitution=0.0,
+                         gas_drag_rate=1e-3, collisions=True)
+    dp = nbx.disk_params(1e-5, nbx.makePerturber(False, 0.0, 5.2, 0.0, 1.0))
+    with pytest.raises(RuntimeError, match="same particle arrays"):
+        nbx.step_concurrent([t2, t3], [p, p], sp, [[dp], [dp]], 0, 1)
 
 
 def test_graph_replay_equals_direct_launches(gpu):
