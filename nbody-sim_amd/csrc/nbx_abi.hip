@@ -740,7 +740,7 @@ int nbx_summarize(nbx_tree *t)
 {
     if (!t) return fail(NBX_E_ARG, "null tree");
     if (t->stage < kBuilt) return fail(NBX_E_STATE, "launchSummarizeKernel before build");
-    if (t->stage == kSummarized) return finish(t);  // this build is summarized already
+    if (t->stage == kSummarized) return finish(t);  // the tree is summarized already
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
     return finish(t);

@@ -1918,7 +1918,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         put(j, code, T.noff[j], cell, rg, nc, pa, pb, body);
     }
     if (j >= ni) return;
-    // round 1: the node's count and its child codes
+    // first round trip: the node's count and its child codes
     const int nk = cnt[j];
     int code[SL];
 #pragma unroll
@@ -1927,7 +1927,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
     const int s = seg_of_internal(T, j);
     const int f0 = T.noff[s] + 8 + 8 * (j - T.ioff[s]);
     const int n_s = T.off[s + 1] - T.off[s];
-    // round 2: every child's cell, range and count, or its body (slots past
+    // second round trip: every child's cell, range and count, or its body (slots past
     // the count read safe places and store nothing)
     float4 cell[SL], body[SL];
     int2 rg[SL];
@@ -1942,7 +1942,7 @@ __global__ __launch_bounds__(kBlock) void k_oct_fill(
         nc[u] = cnt[ic];
         body[u] = bodies[lp];
     }
-    // round 3: the prefix at both ends of every child's range
+    // third round trip: the prefix at both ends of every child's range
     DVec<4> pa[SL], pb[SL];
     const int base = T.off[s] + s;
 #pragma unroll

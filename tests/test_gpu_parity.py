@@ -417,7 +417,7 @@ def test_step_vs_reference_loop(gpu):
     """20 KDK steps of main.cu:85-140 at N=4096 with drag and the perturber:
     positions and velocities within 4e-6 (about 30 ulp at |x| ~ 4 AU,
     |v| ~ 1.4) of the oracle's restated reference loop.  The reference sums
-    tree terms into the star-term accumulator, this build adds the tree sum
+    tree terms into the star-term accumulator, the library adds the tree sum
     to the star term once, so accelerations differ in the last bits."""
     nbx = gpu
     d = disk(4096, mass=1e-5, seed=9)
