@@ -23,19 +23,14 @@ __device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64
 // above, within each half) and of lane 31 (to the upper half) join the rows.
 __device__ inline int wave_incl_add(int v)
 {
-    const int l = __lane_id(), rl = l & 15;
-    int t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);  // row_shr:1
-    if (rl >= 1) v += t;
-    t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);  // row_shr:2
-    if (rl >= 2) v += t;
-    t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);  // row_shr:4
-    if (rl >= 4) v += t;
-    t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);  // row_shr:8
-    if (rl >= 8) v += t;
-    t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);  // row_bcast:15
-    if ((l & 31) >= 16) v += t;
-    t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);  // row_bcast:31
-    if (l >= 32) v += t;
+    // Each term reads 0 where its source lane is outside the row (bound_ctrl)
+    // or where its row is masked off (old = 0), so every step is one DPP add.
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 -> rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 -> rows 2, 3
     return v;
 }
 

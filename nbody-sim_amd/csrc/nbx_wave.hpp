@@ -170,7 +170,8 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const uint64_t starts = ballot64(L.own[lane] != 0);
-        const int j = min(max(__popcll(starts & lanemask_le()) - 1, 0), ne - 1);
+        const int nst = (int)__popcll(starts & lanemask_le()) - 1;
+        const int j = nst < 0 ? 0 : (nst > ne - 1 ? ne - 1 : nst);
         const int ej = __shfl(excl, j, kWave);
         my_rec = -1;
         my_mask = 0;
