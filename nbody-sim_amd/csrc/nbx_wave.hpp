@@ -255,7 +255,11 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
                       "pairs of steps tile the batch; the last prefetch stays inside the staging area");
         // Open masks of the batch: lane k of om_r (bits 32..63 in om_h)
         // receives the mask of the targets that opened record k.  Steps only
-        // record them; the children are pushed once, after the batch.
+        // record them; the children are pushed once, after the batch.  With
+        // at most 16 targets a 32-bit word packs RW = 32 / TG records' masks
+        // (record k in lane k / RW, bits (k % RW) TG ..), so a step writes the
+        // two halves of its ballot as they are.
+        constexpr int RW = TG <= 16 ? 32 / TG : 1;
         int om_r = 0, om_h = 0;
         auto evaluate = [&](Rec(&cur)[U], const uint64_t(&mc)[U], int k0) {
 #pragma unroll
@@ -266,14 +270,21 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
                 // lane mask, with no select in between; the acceptance
                 // compare serves both the pull and the open mask)
                 const uint64_t om = ballot64(mine) & ~ballot64(vis.visit_masked(cur[u], mine));
+                if constexpr (RW > 1) {
+                    // records k0 + u LS .. (LS = 2 RW of them): two words
+                    const int w = (k0 + u * LS) / RW;
+                    om_r = nbx_writelane_i32((int)(unsigned)om, w, om_r);
+                    om_r = nbx_writelane_i32((int)(unsigned)(om >> 32), w + 1, om_r);
+                } else {
 #pragma unroll
-                for (int q = 0; q < LS; ++q) {
-                    // slot q of unroll u holds record k0 + u LS + q (< 64)
-                    const int k = k0 + u * LS + q;
-                    const uint64_t mq = (om >> (q * TG)) & TMASK;
-                    om_r = nbx_writelane_i32((int)(unsigned)mq, k, om_r);
-                    if constexpr (TG > 32)
-                        om_h = nbx_writelane_i32((int)(unsigned)(mq >> 32), k, om_h);
+                    for (int q = 0; q < LS; ++q) {
+                        // slot q of unroll u holds record k0 + u LS + q (< 64)
+                        const int k = k0 + u * LS + q;
+                        const uint64_t mq = (om >> (q * TG)) & TMASK;
+                        om_r = nbx_writelane_i32((int)(unsigned)mq, k, om_r);
+                        if constexpr (TG > 32)
+                            om_h = nbx_writelane_i32((int)(unsigned)(mq >> 32), k, om_h);
+                    }
                 }
             }
         };
@@ -301,6 +312,9 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         {  // push the child blocks of the opened records in record order
            // (record k above record k - 1), lane k writing record k's entry
             // (an entry always holds records: the pop relies on counts >= 1)
+            if constexpr (RW > 1)  // lane k takes record k's mask out of its word
+                om_r = (int)(((unsigned)__shfl(om_r, lane / RW, kWave) >> ((lane % RW) * TG)) &
+                             (unsigned)TMASK);
             const bool opened = lane < nrec && rc > 0 && (om_r != 0 || om_h != 0);
             const uint64_t fl = ballot64(opened);
             if (fl) {
@@ -375,7 +389,10 @@ struct GravityVisitor {
         const float rinv3 = rinv2 * rinv;
         // the pull is selected, not the mass: without softening the own
         // leaf has rinv = inf and m rinv^3 would be 0 inf = NaN
-        const float f = take ? a.w * rinv3 : 0.f;
+        // (with softening rinv is finite: the quadrupole form selects rinv^3
+        // once and derives the pull and its r^-5, r^-7 factors from it)
+        const float rsel = take ? rinv3 : 0.f;
+        const float f = (QUAD && !SELF) ? a.w * rsel : (take ? a.w * rinv3 : 0.f);
         if constexpr (!QUAD) {
             fx += f * dx;
             fy += f * dy;
@@ -387,7 +404,7 @@ struct GravityVisitor {
             const float qx = r.q0.x * dx + r.q0.w * dy + r.q1.x * dz;
             const float qy = r.q0.w * dx + r.q0.y * dy + r.q1.y * dz;
             const float qz = r.q1.x * dx + r.q1.y * dy + r.q0.z * dz;
-            const float r5 = take ? rinv3 * rinv2 : 0.f;
+            const float r5 = SELF ? (take ? rinv3 * rinv2 : 0.f) : rsel * rinv2;
             // (without softening the own leaf has rinv2 = inf: select, 0 inf = NaN)
             const float r7 = SELF ? (take ? r5 * rinv2 : 0.f) : r5 * rinv2;
             const float h = __builtin_fmaf(2.5f * r7, dx * qx + dy * qy + dz * qz, f);
