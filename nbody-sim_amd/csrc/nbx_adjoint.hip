@@ -57,7 +57,7 @@ __device__ inline double wave_sum_d(double v)
 template <int TG>
 struct AdjVisitor {
     static constexpr bool kRecId = true;
-    float bx, by, bz, theta2, eps2;
+    float bx, by, bz, ith2, eps2;  // ith2 = theta^-2 (the walk stages h^2 ith2)
     int self;
     float ux, uy, uz;     // abar of the target
     double gx, gy, gz;    // target side: -sum m_c J(d) abar
@@ -69,7 +69,7 @@ struct AdjVisitor {
         const bool leaf = b.w == 0;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        const bool open = mine && !leaf && !(__int_as_float(b.y) < theta2 * r2);
+        const bool open = mine && !leaf && !(__int_as_float(b.y) < r2);
         const bool take = mine && !open && !(leaf && b.z == self);
         float jx, jy, jz;
         jmul(dx, dy, dz, r2 + eps2, ux, uy, uz, jx, jy, jz);
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_adj_walk(
     const float ux = valid ? (float)ab[i] : 0.f;
     const float uy = valid ? (float)ab[n + i] : 0.f;
     const float uz = valid ? (float)ab[2 * n + i] : 0.f;
-    AdjVisitor<TG> v{b.x, b.y, b.z, theta2, eps2, pos, ux, uy, uz, 0.0, 0.0, 0.0, G};
+    AdjVisitor<TG> v{b.x, b.y, b.z, 1.0f / theta2, eps2, pos, ux, uy, uz, 0.0, 0.0, 0.0, G};
     const uint64_t active = ballot64(valid && rr == 0);
     wave_walk<TG, false>(ra, rb, nullptr, T.noff[s], active, lds[wib], err, v);
     double gx = slot_sum_d<TG>(v.gx), gy = slot_sum_d<TG>(v.gy), gz = slot_sum_d<TG>(v.gz);

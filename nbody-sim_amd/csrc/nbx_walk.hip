@@ -77,7 +77,7 @@ __device__ inline void walk_block(const SegTable &T, const WaveMap &M, const Gra
     const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
-    GravityVisitor<QUAD, SELF> v{b.x, b.y, b.z, theta2, eps2, pos, 0.f, 0.f, 0.f};
+    GravityVisitor<QUAD, SELF> v{b.x, b.y, b.z, 1.0f / theta2, eps2, pos, 0.f, 0.f, 0.f};
     const uint64_t active = ballot64(valid && rr == 0);  // bit tt per valid target
     const int work = wave_walk<TG, QUAD, U>(ra, rb, rq, T.noff[s], active, lds[wib], err, v);
     if (cost && lane == 0) cost[par + w] = work;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kGravBlock) void k_walk_counts(
     const int pos = T.off[s] + (w - M.wave[s]) * TG + tt;
     const bool valid = pos < T.off[s + 1];
     const float4 b = bodies[valid ? pos : T.off[s]];
-    CountVisitor v{b.x, b.y, b.z, theta2, pos, 0, 0};
+    CountVisitor v{b.x, b.y, b.z, 1.0f / theta2, pos, 0, 0};
     wave_walk<TG, false>(ra, rb, nullptr, T.noff[s], ballot64(valid && rr == 0), lds[wib], err, v,
                          wstat ? wstat + 3 * w : nullptr);
     const int na = slot_sum_i<TG>(v.n_acc), no = slot_sum_i<TG>(v.n_open);
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_mix(float *out, int iters
     if (t < 64) {
         const float u = (float)t;
         ta[t] = make_float4(0.7f * u - 20.f, 0.05f * u, 0.3f * u - 9.f, 1e-6f * (u + 1.f));
-        th2[t] = (t & 3) == 0 ? -1.f : 0.01f * u;
+        th2[t] = (t & 3) == 0 ? -1.f : 0.01f * u / theta2;  // staged h^2 theta^-2
     }
     __syncthreads();
     const float bx = 0.01f * t, by = 0.001f * t, bz = -0.01f * t;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_mix(float *out, int iters
         const float4 a = ta[k];
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        const bool open = !(th2[k] < theta2 * r2);
+        const bool open = !(th2[k] < r2);
         const bool take = !open && k != (t & 63);
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float mw = take ? a.w : 0.f;

@@ -217,7 +217,10 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
         {  // stage the batch that arrived; a lane without a record stages the
            // root record under a zero mask, so no slot needs a bound test
             L.s[lane].a = ga;
-            L.s[lane].b = make_int4((int)(unsigned)mask_n, gb.y, gb.z, gb.w);
+            // (h^2 staged times theta^-2: the opening test of every lane and
+            // record is then one compare, h^2 theta^-2 < r^2)
+            L.s[lane].b = make_int4((int)(unsigned)mask_n,
+                                    __float_as_int(__int_as_float(gb.y) * vis.ith2), gb.z, gb.w);
             if constexpr (TG > 32) L.mhi[lane] = (unsigned)(mask_n >> 32);
             if constexpr (wants_rec_id<Visitor>::value) L.id[lane] = rec_n;
             if constexpr (QUAD) {
@@ -357,8 +360,10 @@ __device__ inline int slot_sum_i(int v)
 }
 
 // ---------------------------------------------------------------------------
-// Gravity: accept a record when it is a leaf or h^2 < theta^2 r^2 (the
-// half-width / COM-distance test of BHTree.cu:576-577); skip the target's
+// Gravity: accept a record when it is a leaf or h^2 theta^-2 < r^2 (the
+// half-width / COM-distance test of BHTree.cu:576-577 without the square
+// root and the division; the walk stages h^2 theta^-2, theta^-2 = 1 / (theta
+// theta) rounded once, so a power-of-two theta scales exactly); skip the target's
 // own leaf (BHTree.cu:580); monopole pull m d / (r^2+eps^2)^{3/2}.
 // SELF = false drops the own-leaf test: a leaf record holds its body's
 // position bit for bit, so for the own leaf d = +0 and the pull m d rinv^3
@@ -369,7 +374,7 @@ constexpr float kNoSelfTestEps2 = 1e-12f;
 
 template <bool QUAD, bool SELF = true>
 struct GravityVisitor {
-    float bx, by, bz, theta2, eps2;
+    float bx, by, bz, ith2, eps2;  // ith2 = theta^-2 (the walk stages h^2 ith2)
     int self;
     float fx, fy, fz;
     // Branch-free: every lane evaluates the record and selects the result, so
@@ -381,7 +386,7 @@ struct GravityVisitor {
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
         // a leaf stores h^2 = -1, so it is always accepted
-        const bool acc = __int_as_float(b.y) < theta2 * r2;
+        const bool acc = __int_as_float(b.y) < r2;
         bool take = mine & acc;
         if constexpr (SELF) take = take & !(b.w == 0 && b.z == self);
         const float rinv = __builtin_amdgcn_rsqf(r2 + eps2);
@@ -423,7 +428,7 @@ struct GravityVisitor {
 // Interaction counts of the same walk (accepted records excluding the own
 // leaf, opened internal records): the algorithmic work of the force pass.
 struct CountVisitor {
-    float bx, by, bz, theta2;
+    float bx, by, bz, ith2;  // theta^-2 (the walk stages h^2 ith2)
     int self;
     int n_acc, n_open;
     __device__ inline bool visit_masked(const Rec &r, bool mine)
@@ -436,7 +441,7 @@ struct CountVisitor {
         const int4 &b = r.b;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        const bool open = !leaf && !(__int_as_float(b.y) < theta2 * r2);
+        const bool open = !leaf && !(__int_as_float(b.y) < r2);
         if (open) ++n_open;
         else if (!(leaf && b.z == self)) ++n_acc;
         return open;

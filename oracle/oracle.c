@@ -735,6 +735,13 @@ static inline float mt_dist2(float dx, float dy, float dz)
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
+/* Opening test of the compressed walk: h^2 theta^-2 < r^2 with theta^-2 =
+ * 1 / (theta theta) rounded once (the reference's h / r < theta,
+ * BHTree.cu:576-577, without the square root and the division; for theta
+ * a power of two, e.g. 0.5 or 0.25, both scalings are exact). */
+static inline float mt_inv_theta2(float theta) { return 1.0f / (theta * theta); }
+static inline int mt_accept(float h2, float ith2, float r2) { return h2 * ith2 < r2; }
+
 typedef struct {
     const uint64_t *keys;
     const float *sx, *sy, *sz, *sm; /* bodies in sorted order */
@@ -887,7 +894,7 @@ int orc_mt_force_at(const float *x, const float *y, const float *z, const float 
     T.child = (int *)malloc((size_t)2 * (n > 1 ? n : 1) * sizeof(int));
     T.rhs = root[3];
     if (n > 1) mt_build_rec(&T, 0, n);
-    const float theta2 = theta * theta, eps2 = softening * softening;
+    const float ith2 = mt_inv_theta2(theta), eps2 = softening * softening;
     int overflow = 0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
@@ -907,7 +914,7 @@ int orc_mt_force_at(const float *x, const float *y, const float *z, const float 
                 const mt_node *r = &T.nodes[0];
                 float dx = r->cx - bx, dy = r->cy - by, dz = r->cz - bz;
                 float r2 = mt_dist2(dx, dy, dz);
-                if (r->h2 < theta2 * r2) {
+                if (mt_accept(r->h2, ith2, r2)) {
                     float rinv = 1.0f / sqrtf(r2 + eps2);
                     float f = r->m * rinv * rinv * rinv;
                     fx += f * dx; fy += f * dy; fz += f * dz;
@@ -934,7 +941,7 @@ int orc_mt_force_at(const float *x, const float *y, const float *z, const float 
                     }
                     float dx = cxn - bx, dy = cyn - by, dz = czn - bz;
                     float r2 = mt_dist2(dx, dy, dz);
-                    if (leaf || T.nodes[c].h2 < theta2 * r2) {
+                    if (leaf || mt_accept(T.nodes[c].h2, ith2, r2)) {
                         float rinv = 1.0f / sqrtf(r2 + eps2);
                         float f = cm * rinv * rinv * rinv;
                         fx += f * dx; fy += f * dy; fz += f * dz;
@@ -988,7 +995,7 @@ long orc_mt_lists(const float *x, const float *y, const float *z, const float *m
     T.child = (int *)malloc((size_t)2 * (n > 1 ? n : 1) * sizeof(int));
     T.rhs = root[3];
     if (n > 1) mt_build_rec(&T, 0, n);
-    const float theta2 = theta * theta;
+    const float ith2 = mt_inv_theta2(theta);
     int32_t *cnt = (int32_t *)calloc((size_t)n, sizeof(int32_t));
     const int fill = lo && hi;
     long total = 0;
@@ -1008,7 +1015,7 @@ long orc_mt_lists(const float *x, const float *y, const float *z, const float *m
             int top = 0;
             const mt_node *r = &T.nodes[0];
             float dx = r->cx - bx, dy = r->cy - by, dz = r->cz - bz;
-            if (r->h2 < theta2 * mt_dist2(dx, dy, dz)) {
+            if (mt_accept(r->h2, ith2, mt_dist2(dx, dy, dz))) {
                 if (pass == 1) { lo[w] = r->lo; hi[w] = r->hi; ++w; }
                 ++c_i;
             } else {
@@ -1027,7 +1034,7 @@ long orc_mt_lists(const float *x, const float *y, const float *z, const float *m
                     }
                     const mt_node *nd = &T.nodes[c];
                     dx = nd->cx - bx; dy = nd->cy - by; dz = nd->cz - bz;
-                    if (nd->h2 < theta2 * mt_dist2(dx, dy, dz)) {
+                    if (mt_accept(nd->h2, ith2, mt_dist2(dx, dy, dz))) {
                         if (pass == 1) { lo[w] = nd->lo; hi[w] = nd->hi; ++w; }
                         ++c_i;
                     } else if (top < 512) {

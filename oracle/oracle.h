@@ -152,7 +152,7 @@ void orc_mt_keys(const float *x, const float *y, const float *z, int n, const fl
 void orc_mt_sort(const uint64_t *keys, int n, int32_t *perm);
 
 /* Compressed octree of the sorted keys and the per-body walk with the
- * reference opening rule written as h^2 < theta^2 r^2.  Tree terms are
+ * reference opening rule written as h^2 theta^-2 < r^2.  Tree terms are
  * accumulated from zero (self-gravity), the star term is returned apart.
  * n_acc / n_open as in orc_ref_force (n_open counts compressed nodes).
  * quad=1 adds the traceless quadrupole of each accepted internal node. */
