@@ -202,11 +202,15 @@ __device__ inline int wave_walk(const float4 *__restrict__ ra, const int4 *__res
     auto fetch = [&]() {
         pop(nrec_n, rec_n, mask_n);
         const int ri = rec_n >= 0 ? rec_n : root;
-        ga = ra[ri];
-        gb = rb[ri];
+        // 32-bit byte offsets (records < 2^27) keep the loads on a scalar
+        // base plus one vector offset
+        const unsigned o16 = (unsigned)ri << 4;
+        ga = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(ra) + o16);
+        gb = *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(rb) + o16);
         if constexpr (QUAD) {
-            gq0 = rq[2 * ri];
-            gq1 = rq[2 * ri + 1];
+            const char *q = reinterpret_cast<const char *>(rq) + 2 * o16;
+            gq0 = *reinterpret_cast<const float4 *>(q);
+            gq1 = *reinterpret_cast<const float4 *>(q + 16);
         }
     };
     fetch();
