@@ -962,7 +962,10 @@ int nbx_fof_disks(nbx_tree *t, const nbx_particles *p, const float *link_lengths
     for (int s = 0; s < t->T.nseg; ++s) LT.ll[s] = link_lengths[s];
     launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
     launch_root(t->T, t->B, t->stream);
-    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    // the bucket splitters of the tree's last build serve any positions (a
+    // bucket they misfit is sorted in global memory), so FoF after steps
+    // does not sample new ones
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, !t->B.split_ready);
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
     launch_fof(t->T, LT, dev_labels, t->B, t->stream);
@@ -1428,7 +1431,7 @@ int adj_force(nbx_tree *t, const nbx_particles *p, const AdjWalkArgs &A, double 
 {
     launch_bbox(t->T, p->x, p->y, p->z, t->B, t->stream);
     launch_root(t->T, t->B, t->stream);
-    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream);
+    launch_build(t->T, p->x, p->y, p->z, p->mass, t->B, t->stream, !t->B.split_ready);
     launch_summarize(t->T, t->B, t->stream);
     t->stage = kSummarized;
     launch_adj_force(t->T, A, t->B, t->stream);
