@@ -389,7 +389,7 @@ struct GravityVisitor {
         const int4 &b = r.b;
         const float dx = a.x - bx, dy = a.y - by, dz = a.z - bz;
         const float r2 = dist2(dx, dy, dz);
-        // a leaf stores h^2 = -1, so it is always accepted
+        // a leaf stores h^2 = -1 (staged negative), so it is always accepted
         const bool acc = __int_as_float(b.y) < r2;
         bool take = mine & acc;
         if constexpr (SELF) take = take & !(b.w == 0 && b.z == self);
