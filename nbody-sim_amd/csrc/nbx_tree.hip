@@ -1668,10 +1668,11 @@ __device__ inline void karras_node(const SegTable &T, const uint64_t *__restrict
 }
 
 // One launch for two independent passes over the sorted order: the first
-// ntile blocks are the moment prefix (and the next ntile the second
-// moments, with quadrupoles), each taking its tiles by ticket in dispatch
-// order, so a tile waits only on tiles already running; the other blocks
-// build the binary radix tree, which never waits.
+// blocks build the binary radix tree, which never waits (dispatched first:
+// its nodes with long ranges hold the longest chains of dependent reads);
+// the next ntile blocks are the moment prefix (and the next ntile the
+// second moments, with quadrupoles), each taking its tiles by ticket in
+// dispatch order, so a tile waits only on tiles already running.
 template <int KIND0>
 __global__ __launch_bounds__(kBlock) void k_prefix_karras(
     SegTable T, const float4 *__restrict__ bodies, DVec<4> *P, DVec<4> *pagg, DVec<4> *pinc,
@@ -1680,19 +1681,18 @@ __global__ __launch_bounds__(kBlock) void k_prefix_karras(
     const uint64_t *__restrict__ keys, const Root *__restrict__ roots, int *left, int *right,
     int *plen, int *octf, float4 *bin_cell, int2 *brange)
 {
-    const int b = blockIdx.x;
+    const int ntree = (T.ioff[T.nseg] + kBlock - 1) / kBlock;
+    if ((int)blockIdx.x < ntree) {
+        karras_node(T, keys, roots, left, right, plen, octf, bin_cell, brange,
+                    (int)blockIdx.x * kBlock + (int)threadIdx.x);
+        return;
+    }
+    const int b = (int)blockIdx.x - ntree;
     if (b < ntile) {
         prefix_tile<4, KIND0>(T, bodies, P, pagg, pinc, tickets, kTicketScan, ghist, s64, perm);
         return;
     }
-    const int nscan = P2 ? 2 * ntile : ntile;
-    if (b < nscan) {
-        prefix_tile<6, 1>(T, bodies, P2, pagg2, pinc2, tickets, kTicketScan2, ghist, nullptr,
-                          nullptr);
-        return;
-    }
-    karras_node(T, keys, roots, left, right, plen, octf, bin_cell, brange,
-                (b - nscan) * kBlock + (int)threadIdx.x);
+    prefix_tile<6, 1>(T, bodies, P2, pagg2, pinc2, tickets, kTicketScan2, ghist, nullptr, nullptr);
 }
 
 // Mass and COM of sorted positions [q.x, q.y] of disk s from the moment
