@@ -31,7 +31,7 @@ constexpr int kRecPad = 16;      // records of padding after the node array
 // Splitter-bucket key sort (disks of at most kSplitMaxN bodies).
 constexpr int kSplitItems = 2;                     // bodies per thread of a scatter block
 constexpr int kSplitTile = kBlock * kSplitItems;   // bodies per scatter block
-constexpr int kBucketMean = 256;                   // target bodies per bucket
+constexpr int kBucketMean = 320;                   // target bodies per bucket
 constexpr int kMaxBuckets = 1024;                  // buckets per disk
 constexpr int kOversample = 8;                     // samples per bucket
 constexpr int kMaxSample = kOversample * kMaxBuckets;
